@@ -1,0 +1,73 @@
+/*
+ * rhydro_driver.h -- host-side driver helpers of the standalone build.
+ *
+ * These have no counterpart among the reference's hydro virtuals: they stand
+ * in for the parts of the reference's Simulation / TreeUpdater / Initializer
+ * that a run needs around the hot path (uniform mesh construction, region
+ * and seeded initial conditions) and expose the per-kernel timers used by
+ * the benchmark.  A drop-in user keeps the reference's own mesh and
+ * initialisation and needs none of this.
+ */
+#ifndef RHYDRO_DRIVER_H
+#define RHYDRO_DRIVER_H
+
+#include "rhydro.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Number of grids a uniform mesh of nx*ny*nz coarse cells refined down to
+ * level list `levelmin` needs (the reference's numbering: level list L holds
+ * ncoarse*8^(L-2) octs). */
+long long rh_uniform_ngrid(int nx, int ny, int nz, int levelmin);
+
+/* Builds the uniform mesh the reference builds for levelmin == levelmax
+ * (Simulation.cpp:226-234 with TreeUpdater::make_grid_fine): grids numbered
+ * 1,2,3,... breadth first, children in cell order, level list L+1 filled from
+ * level list L in list order.  Arrays are caller-allocated, sized as in
+ * AmrGrid::allocate (AmrGrid.cpp:13-70) for the given ngridmax, and fully
+ * written here: father[ngridmax], son[ncell], nbor[6*ngridmax],
+ * next/prev[ngridmax] (level lists, then the free list from the first unused
+ * grid), headl/taill/numbl[nlevelmax+1] (ncpu = 1), xg[3*ngridmax].
+ * Returns the number of grids used or a negative rh_status. */
+long long rh_build_uniform(int nx, int ny, int nz, int levelmin, int nlevelmax, int ngridmax, int* father,
+                           int* son, int* nbor, int* next, int* prev, int* headl, int* taill, int* numbl,
+                           double* xg);
+
+/* Region initial conditions (Initializer.cpp:31-157, types 'square'/'cube'
+ * and 'sphere'), applied to every cell of every level list and the coarse
+ * cells, then restricted upward as Simulation::initialize's final
+ * legacy_init_flow does (Simulation.cpp:218-224,257; TreeUpdater.cpp:195-215).
+ * Region arrays have nreg entries; type[i] is 0 = square/cube, 1 = sphere.
+ * unew is set equal to uold. */
+int rh_init_regions(const rh_mesh* mesh, const double* xg, double boxlen, double gamma, int nreg, const int* type,
+                    const double* x_center, const double* y_center, const double* z_center,
+                    const double* length_x, const double* length_y, const double* length_z, const double* d_region,
+                    const double* u_region, const double* v_region, const double* w_region,
+                    const double* p_region);
+
+/* Seeded random state on the leaf cells of level list `level`, in list order
+ * and cell order: rho = exp(s*Z1), p = exp(s*Z2), v_i = a*c_s*Z with
+ * c_s = sqrt(gamma*p/rho), E = p/(gamma-1) + rho*v^2/2, where the Z are
+ * standard normals from std::mt19937_64(seed) via Box-Muller (pairs of
+ * draws u1,u2 in (0,1]: sqrt(-2 ln u1) * cos(2 pi u2), then sin).  Every
+ * other cell of the mesh gets rho=1, v=0, p=1.  unew = uold. */
+int rh_init_lognormal(const rh_mesh* mesh, int level, unsigned long long seed, double gamma, double sigma,
+                      double vel_amp);
+
+/* Per-kernel device time accumulated on the context's stream (HIP events
+ * around each launch) since the last reset: kernel 0 = godunov,
+ * 1 = courant, 2 = set_unew, 3 = set_uold.  ms_total/count per kernel. */
+int rh_timers_enable(rh_ctx* ctx, int enable);
+int rh_timers_read(rh_ctx* ctx, int kernel, double* ms_total, long long* count);
+int rh_timers_reset(rh_ctx* ctx);
+
+/* Leaf cells of a level in the bound mesh (cell-updates per godunov call). */
+int rh_level_leaf_cells(rh_ctx* ctx, int ilevel, long long* nleaf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RHYDRO_DRIVER_H */

@@ -1,0 +1,209 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and of the reference's own
+physics sources compiled in place (_ref/libref_kernels.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref_kernels.so")
+
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+
+class OrMesh(C.Structure):
+    _fields_ = [
+        ("ncoarse", C.c_int), ("ngridmax", C.c_int), ("ncell", C.c_int), ("nvar", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("nlevelmax", C.c_int), ("ncpu", C.c_int), ("nboundary", C.c_int), ("myid", C.c_int),
+        ("father", _IP), ("son", _IP), ("nbor", _IP), ("next", _IP), ("headl", _IP),
+        ("uold", _DP), ("unew", _DP), ("f", _DP),
+        ("bound_type", _IP), ("nbound_type", C.c_int),
+    ]
+
+
+class OrParams(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("slope_type", C.c_int), ("riemann", C.c_int),
+        ("interpol_type", C.c_int), ("interpol_var", C.c_int), ("boxlen", C.c_double),
+    ]
+
+
+def build(quiet: bool = True) -> None:
+    """Compiles liboracle.so (and _ref when the reference tree is present)."""
+    out = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+
+
+_lib = None
+_ref = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.or_godunov_fine.argtypes = [C.POINTER(OrMesh), C.POINTER(OrParams), C.c_int, C.c_double, C.c_double]
+        L.or_godunov_fine_sample.argtypes = [C.POINTER(OrMesh), C.POINTER(OrParams), C.c_int, C.c_double,
+                                             C.c_double, C.c_longlong]
+        L.or_set_unew.argtypes = [C.POINTER(OrMesh), C.c_int]
+        L.or_set_uold.argtypes = [C.POINTER(OrMesh), C.c_int]
+        L.or_courant.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.or_courant.restype = C.c_double
+        L.or_interpol_hydro.argtypes = [C.POINTER(OrParams), C.c_int, _DP, _DP]
+        L.or_ctoprim.argtypes = [_DP, _DP, C.c_double]
+        L.or_trace.argtypes = [_DP, _DP, C.c_double, C.c_double, _DP, _DP, C.c_double]
+        L.or_compute_slope.argtypes = [C.c_double] * 3 + [C.c_int, C.c_double, C.c_double]
+        L.or_compute_slope.restype = C.c_double
+        L.or_ghost_slope.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.or_ghost_slope.restype = C.c_double
+        L.or_get_cs2.argtypes = [C.c_double] * 3
+        L.or_get_cs2.restype = C.c_double
+        L.or_riemann.argtypes = [C.c_int, _DP, _DP, _DP, C.c_double]
+        L.or_prim_to_cons.argtypes = [_DP, _DP, C.c_double]
+        L.or_compute_flux.argtypes = [_DP, _DP, C.c_double]
+        L.or_build_uniform.argtypes = [C.c_int] * 6 + [_IP] * 8
+        _lib = L
+    return _lib
+
+
+def ref() -> C.CDLL | None:
+    """The reference's Riemann/slope/EOS sources (None when not built)."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_PATH):
+            return None
+        R = C.CDLL(REF_PATH)
+        R.ref_riemann.argtypes = [C.c_int, _DP, _DP, _DP, C.c_double]
+        R.ref_compute_slope.argtypes = [C.c_double] * 3 + [C.c_int, C.c_double, C.c_double]
+        R.ref_compute_slope.restype = C.c_double
+        R.ref_get_cs2.argtypes = [C.c_double] * 3
+        R.ref_get_cs2.restype = C.c_double
+        _ref = R
+    return _ref
+
+
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_IP)
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+def params(gamma=1.4, slope_type=1, riemann=0, interpol_type=1, interpol_var=0, boxlen=1.0) -> OrParams:
+    return OrParams(gamma, slope_type, riemann, interpol_type, interpol_var, boxlen)
+
+
+class OracleMesh:
+    """Oracle view over AmrGrid-shaped numpy arrays (any object exposing
+    father/son/nbor/next/headl/uold/unew and the scalars)."""
+
+    def __init__(self, m, f: np.ndarray | None = None):
+        self.m = m
+        self._bt = np.zeros(1, np.int32)
+        s = OrMesh()
+        s.ncoarse, s.ngridmax, s.ncell, s.nvar = m.ncoarse, m.ngridmax, m.ncell, m.nvar
+        s.nx, s.ny, s.nz = m.nx, m.ny, m.nz
+        s.nlevelmax, s.ncpu, s.nboundary, s.myid = m.nlevelmax, 1, 0, 1
+        s.father, s.son, s.nbor, s.next, s.headl = (_ip(getattr(m, k)) for k in
+                                                    ("father", "son", "nbor", "next", "headl"))
+        s.uold, s.unew = _dp(m.uold), _dp(m.unew)
+        self._f = f
+        s.f = _dp(f) if f is not None else C.cast(None, _DP)
+        s.bound_type, s.nbound_type = _ip(self._bt), 0
+        self.s = s
+
+    def godunov_fine(self, p: OrParams, ilevel, dt, dx):
+        st = lib().or_godunov_fine(C.byref(self.s), C.byref(p), ilevel, dt, dx)
+        if st != 0:
+            raise RuntimeError(f"oracle godunov_fine status {st}")
+
+    def godunov_fine_sample(self, p: OrParams, ilevel, dt, dx, max_octs):
+        st = lib().or_godunov_fine_sample(C.byref(self.s), C.byref(p), ilevel, dt, dx, max_octs)
+        if st != 0:
+            raise RuntimeError(f"oracle godunov_fine status {st}")
+
+    def set_unew(self, ilevel):
+        lib().or_set_unew(C.byref(self.s), ilevel)
+
+    def set_uold(self, ilevel):
+        lib().or_set_uold(C.byref(self.s), ilevel)
+
+    def courant(self, ilevel, dx, gamma, courant_factor, boxlen=1.0) -> float:
+        return lib().or_courant(C.byref(self.s), ilevel, dx, gamma, courant_factor, boxlen)
+
+    def run_uniform(self, p: OrParams, ilevel, dx, courant_factor, nsteps) -> np.ndarray:
+        """Simulation::amr_step(ilevel, 1) with levelmin == levelmax
+        (Simulation.cpp:522-635): courant, set_unew, godunov_fine, set_uold."""
+        dts = []
+        for _ in range(nsteps):
+            dt = self.courant(ilevel, dx, p.gamma, courant_factor, p.boxlen)
+            self.set_unew(ilevel)
+            self.godunov_fine(p, ilevel, dt, dx)
+            self.set_uold(ilevel)
+            dts.append(dt)
+        return np.array(dts)
+
+
+def interpol(p: OrParams, u1: np.ndarray, nvar: int = 5) -> np.ndarray:
+    u1 = np.ascontiguousarray(u1, np.float64).reshape(7, 64)
+    u2 = np.zeros((8, 64), np.float64)
+    lib().or_interpol_hydro(C.byref(p), nvar, _dp(u1), _dp(u2))
+    return u2
+
+
+def riemann(kind: int, ql, qr, gamma: float, use_ref: bool = False) -> np.ndarray:
+    ql = np.ascontiguousarray(ql, np.float64)
+    qr = np.ascontiguousarray(qr, np.float64)
+    f = np.zeros(5, np.float64)
+    if use_ref:
+        ref().ref_riemann(kind, _dp(ql), _dp(qr), _dp(f), gamma)
+    else:
+        lib().or_riemann(kind, _dp(ql), _dp(qr), _dp(f), gamma)
+    return f
+
+
+def ctoprim(u, gamma):
+    u = np.ascontiguousarray(u, np.float64)
+    q = np.zeros(5)
+    lib().or_ctoprim(_dp(u), _dp(q), gamma)
+    return q
+
+
+def trace(q, dq, dt, dx, gamma):
+    q = np.ascontiguousarray(q, np.float64)
+    dq = np.ascontiguousarray(dq, np.float64)
+    qm, qp = np.zeros(5), np.zeros(5)
+    lib().or_trace(_dp(q), _dp(dq), dt, dx, _dp(qm), _dp(qp), gamma)
+    return qm, qp
+
+
+def build_uniform(nx, ny, nz, levelmin, nlevelmax, ngridmax):
+    """The reference's own construction sequence (see rh_oracle.cpp)."""
+    ncoarse = nx * ny * nz
+    ncell = ncoarse + 8 * ngridmax
+    a = dict(father=np.zeros(ngridmax, np.int32), son=np.zeros(ncell, np.int32),
+             nbor=np.zeros(6 * ngridmax, np.int32), next=np.zeros(ngridmax, np.int32),
+             prev=np.zeros(ngridmax, np.int32), headl=np.zeros(nlevelmax + 1, np.int32),
+             taill=np.zeros(nlevelmax + 1, np.int32), numbl=np.zeros(nlevelmax + 1, np.int32))
+    used = lib().or_build_uniform(nx, ny, nz, levelmin, nlevelmax, ngridmax,
+                                  *[_ip(a[k]) for k in ("father", "son", "nbor", "next", "prev", "headl", "taill",
+                                                        "numbl")])
+    if used < 0:
+        raise RuntimeError("ngridmax too small")
+    return a, used

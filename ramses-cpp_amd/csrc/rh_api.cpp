@@ -1,0 +1,551 @@
+// rh_api.cpp -- the C-ABI (rhydro.h): context, binding, residency and the
+// per-call orchestration of the level kernels.  Host code; kernels live in
+// rh_hydro.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rh_kernels.h"
+#include "rh_tables.h"
+#include "rhydro.h"
+#include "rhydro_driver.h"
+
+static_assert(rh::REC_REFINED == rh::REC_REFINED_DEV, "record flags");
+static_assert(rh::REC_BOUNDARY == rh::REC_BOUNDARY_DEV, "record flags");
+static_assert(rh::REC_REFLUX == rh::REC_REFLUX_DEV, "record flags");
+
+namespace {
+
+std::string g_create_error;
+
+struct DevLevel {
+    long long epoch = -2;
+    rh::LevelDev dev;
+    std::vector<void*> allocs;
+};
+
+enum { TK_GODUNOV = 0, TK_COURANT = 1, TK_SET_UNEW = 2, TK_SET_UOLD = 3, TK_N = 4 };
+
+}  // namespace
+
+struct rh_ctx {
+    rh_params p{};
+    int mode = RH_HOST_COHERENT;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool bound = false;
+    rh_mesh mesh{};
+    double* d_uold = nullptr;
+    double* d_unew = nullptr;
+    double* d_f = nullptr;
+    size_t state_cap = 0;  // doubles per state array
+    size_t f_cap = 0;
+    rh::TopoIndex topo;
+    bool topo_valid = false;
+    std::vector<DevLevel> levels;
+    double* d_part = nullptr;
+    int npart = 0;
+    double* d_dt = nullptr;
+    int ndt = 0;
+    double* d_u1 = nullptr;
+    double* d_u2 = nullptr;
+    int interpol_cap = 0;
+    bool timers = false;
+    double t_ms[TK_N] = {0, 0, 0, 0};
+    long long t_n[TK_N] = {0, 0, 0, 0};
+    // Event pairs recorded around launches, summed when the timers are read.
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+};
+
+namespace {
+
+int fail(rh_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int hip_fail(rh_ctx* c, hipError_t e, const char* what) {
+    return fail(c, RH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define RH_HIP(c, call)                                    \
+    do {                                                   \
+        hipError_t _e = (call);                            \
+        if (_e != hipSuccess) return hip_fail(c, _e, #call); \
+    } while (0)
+
+void free_level(DevLevel& L) {
+    for (void* p : L.allocs) (void)hipFree(p);
+    L.allocs.clear();
+    L.dev = rh::LevelDev{};
+    L.epoch = -2;
+}
+
+template <typename T> hipError_t upload(DevLevel& L, T*& dst, const std::vector<T>& src, hipStream_t s) {
+    dst = nullptr;
+    if (src.empty()) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, src.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    L.allocs.push_back(p);
+    dst = (T*)p;
+    e = hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    return e;
+}
+
+rh::MeshDev mesh_dev(const rh_ctx* c) {
+    rh::MeshDev m;
+    m.ncoarse = c->mesh.ncoarse;
+    m.ngridmax = c->mesh.ngridmax;
+    m.ncell = c->mesh.ncell;
+    m.nx = c->mesh.nx;
+    m.ny = c->mesh.ny;
+    m.nz = c->mesh.nz;
+    m.uold = c->d_uold;
+    m.unew = c->d_unew;
+    m.f = c->d_f;
+    return m;
+}
+
+rh::GasParams gas(const rh_ctx* c, double gamma) {
+    rh::GasParams g;
+    g.gamma = gamma;
+    g.gm1 = gamma - 1.0;
+    double smallc = 1e-10;
+    g.smalle = smallc * smallc / gamma / (gamma - 1.0);
+    g.smallp = smallc * smallc / gamma;
+    g.slope_type = c->p.slope_type;
+    g.interpol_type = c->p.interpol_type;
+    return g;
+}
+
+// Level tables for ilevel, rebuilt when the mesh epoch moved.
+int get_level(rh_ctx* c, int ilevel, const rh::LevelDev** out) {
+    if (ilevel < 1 || ilevel > c->mesh.nlevelmax) return fail(c, RH_ERR_ARG, "ilevel out of range");
+    if (!c->topo_valid || c->topo.epoch != c->mesh.mesh_epoch) {
+        std::string e;
+        int st = rh::build_topo_index(c->mesh, c->topo, e);
+        if (st != RH_OK) return fail(c, st, e);
+        c->topo_valid = true;
+    }
+    if ((int)c->levels.size() <= ilevel) c->levels.resize(ilevel + 1);
+    DevLevel& L = c->levels[ilevel];
+    if (L.epoch != c->mesh.mesh_epoch) {
+        free_level(L);
+        rh::LevelHost h;
+        std::string e;
+        int st = rh::build_level(c->mesh, c->topo, ilevel, h, e);
+        if (st != RH_OK) return fail(c, st, e);
+        rh::LevelDev& d = L.dev;
+        d.noct = h.noct;
+        d.nrec = (int)h.rec_n.size();
+        d.nslots = h.nslots;
+        d.ntgt = (int)h.tgt_cell.size();
+        d.nleaf = h.nleaf;
+        std::vector<int4> rec(d.nrec);
+        for (int r = 0; r < d.nrec; ++r) rec[r] = make_int4(h.rec_n[r], h.rec_nl[r], h.rec_nr[r], h.rec_flags[r]);
+        RH_HIP(c, upload(L, d.grid, h.grid, c->stream));
+        RH_HIP(c, upload(L, d.leaf, h.leaf, c->stream));
+        RH_HIP(c, upload(L, d.nbo, h.nbo, c->stream));
+        RH_HIP(c, upload(L, d.rec, rec, c->stream));
+        RH_HIP(c, upload(L, d.rec_slot, h.rec_slot, c->stream));
+        RH_HIP(c, upload(L, d.tgt_cell, h.tgt_cell, c->stream));
+        RH_HIP(c, upload(L, d.tgt_off, h.tgt_off, c->stream));
+        RH_HIP(c, upload(L, d.tgt_slot, h.tgt_slot, c->stream));
+        if (d.nslots > 0) {
+            void* p = nullptr;
+            RH_HIP(c, hipMalloc(&p, (size_t)d.nslots * 5 * sizeof(double)));
+            L.allocs.push_back(p);
+            d.slot_r = (double*)p;
+        }
+        RH_HIP(c, hipStreamSynchronize(c->stream));  // host vectors go out of scope
+        L.epoch = c->mesh.mesh_epoch;
+    }
+    *out = &L.dev;
+    return RH_OK;
+}
+
+int ensure_part(rh_ctx* c, int n) {
+    if (n <= c->npart) return RH_OK;
+    if (c->d_part) (void)hipFree(c->d_part);
+    c->d_part = nullptr;
+    RH_HIP(c, hipMalloc(&c->d_part, (size_t)n * sizeof(double)));
+    c->npart = n;
+    return RH_OK;
+}
+
+int ensure_dt(rh_ctx* c, int n) {
+    if (n <= c->ndt) return RH_OK;
+    if (c->d_dt) (void)hipFree(c->d_dt);
+    c->d_dt = nullptr;
+    RH_HIP(c, hipMalloc(&c->d_dt, (size_t)n * sizeof(double)));
+    c->ndt = n;
+    return RH_OK;
+}
+
+size_t state_bytes(const rh_ctx* c) { return (size_t)c->mesh.nvar * c->mesh.ncell * sizeof(double); }
+
+int upload_state(rh_ctx* c, int what) {
+    if (what & RH_UOLD)
+        RH_HIP(c, hipMemcpyAsync(c->d_uold, c->mesh.uold, state_bytes(c), hipMemcpyHostToDevice, c->stream));
+    if (what & RH_UNEW)
+        RH_HIP(c, hipMemcpyAsync(c->d_unew, c->mesh.unew, state_bytes(c), hipMemcpyHostToDevice, c->stream));
+    return RH_OK;
+}
+
+int download_state(rh_ctx* c, int what) {
+    if (what & RH_UOLD)
+        RH_HIP(c, hipMemcpyAsync(c->mesh.uold, c->d_uold, state_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    if (what & RH_UNEW)
+        RH_HIP(c, hipMemcpyAsync(c->mesh.unew, c->d_unew, state_bytes(c), hipMemcpyDeviceToHost, c->stream));
+    RH_HIP(c, hipStreamSynchronize(c->stream));
+    return RH_OK;
+}
+
+int check_bound(rh_ctx* c) {
+    if (!c) return RH_ERR_ARG;
+    if (!c->bound) return fail(c, RH_ERR_ARG, "no mesh bound (call rh_bind first)");
+    return RH_OK;
+}
+
+// In host-coherent mode the host may have changed uold/unew since the last
+// call: copy them in first.
+int pre_call(rh_ctx* c) {
+    if (c->mode == RH_HOST_COHERENT) return upload_state(c, RH_UOLD | RH_UNEW);
+    return RH_OK;
+}
+
+int post_call(rh_ctx* c, int what) {
+    if (c->mode == RH_HOST_COHERENT) return download_state(c, what);
+    return RH_OK;
+}
+
+hipEvent_t take_event(rh_ctx* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Sums the elapsed time of every recorded pair (waits for the stream).
+void collect_timers(rh_ctx* c) {
+    if (c->ev_pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& pe : c->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+            c->t_ms[pe.first] += ms;
+            c->t_n[pe.first] += 1;
+        }
+        c->ev_pool.push_back(pe.second.first);
+        c->ev_pool.push_back(pe.second.second);
+    }
+    c->ev_pending.clear();
+}
+
+struct Timed {
+    rh_ctx* c;
+    int k;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Timed(rh_ctx* cc, int kk) : c(cc), k(kk) {
+        if (!c->timers) return;
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (e0) (void)hipEventRecord(e0, c->stream);
+    }
+    void done() {
+        if (!c->timers || !e0 || !e1) return;
+        (void)hipEventRecord(e1, c->stream);
+        c->ev_pending.push_back({k, {e0, e1}});
+        if (c->ev_pending.size() > 4096) collect_timers(c);
+    }
+};
+
+int do_courant(rh_ctx* c, const rh::LevelDev* lv, double dx, double gamma, double cf, double* d_out) {
+    int st = ensure_part(c, rh::courant_blocks(*lv) + 1);
+    if (st) return st;
+    double dt_init = c->p.boxlen / 1e-10;
+    Timed tm(c, TK_COURANT);
+    RH_HIP(c, rh::launch_courant(mesh_dev(c), *lv, dx, gamma, cf, dt_init, c->d_part, c->npart, d_out, c->stream));
+    tm.done();
+    return RH_OK;
+}
+
+int do_godunov(rh_ctx* c, const rh::LevelDev* lv, const double* d_dt, double dx) {
+    Timed tm(c, TK_GODUNOV);
+    hipError_t e = rh::launch_godunov(mesh_dev(c), *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision, d_dt, dx,
+                                      c->stream);
+    if (e == hipErrorNotSupported) return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
+    if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
+    tm.done();
+    return RH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rh_create(const rh_params* params, rh_ctx** out) {
+    if (!params || !out) return fail(nullptr, RH_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (params->nvar != 5 || params->nener != 0)
+        return fail(nullptr, RH_ERR_UNSUPPORTED, "only nvar=5 (NDIM=3, nener=0, no passive scalars) is supported");
+    if (params->barotropic_eos) return fail(nullptr, RH_ERR_UNSUPPORTED, "barotropic EOS is not supported");
+    if (params->slope_type < 0 || params->slope_type > 8) return fail(nullptr, RH_ERR_ARG, "slope_type outside 0..8");
+    if (params->riemann < 0 || params->riemann > 4) return fail(nullptr, RH_ERR_ARG, "unknown riemann id");
+    if (params->riemann == RH_RIEMANN_EXACT)
+        return fail(nullptr, RH_ERR_UNSUPPORTED, "the exact (Newton-Raphson) Riemann solver is not on the GPU path");
+    if (params->precision != RH_PREC_FP64 && params->precision != RH_PREC_MIXED)
+        return fail(nullptr, RH_ERR_ARG, "unknown precision");
+    if (!(params->gamma > 1.0)) return fail(nullptr, RH_ERR_ARG, "gamma must exceed 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, RH_ERR_NO_DEVICE, "no HIP device available");
+    if (params->device < 0 || params->device >= ndev) return fail(nullptr, RH_ERR_ARG, "device ordinal out of range");
+    std::unique_ptr<rh_ctx> c(new rh_ctx());
+    c->p = *params;
+    hipError_t e = hipSetDevice(params->device);
+    if (e != hipSuccess) return fail(nullptr, RH_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(nullptr, RH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    *out = c.release();
+    return RH_OK;
+}
+
+int rh_destroy(rh_ctx* c) {
+    if (!c) return RH_OK;
+    (void)hipSetDevice(c->p.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& L : c->levels) free_level(L);
+    for (void* p : {(void*)c->d_uold, (void*)c->d_unew, (void*)c->d_f, (void*)c->d_part, (void*)c->d_dt,
+                    (void*)c->d_u1, (void*)c->d_u2})
+        if (p) (void)hipFree(p);
+    collect_timers(c);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RH_OK;
+}
+
+int rh_bind(rh_ctx* c, const rh_mesh* m, int mode) {
+    if (!c || !m) return RH_ERR_ARG;
+    if (mode != RH_HOST_COHERENT && mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "unknown mode");
+    if (m->nvar != 5) return fail(c, RH_ERR_UNSUPPORTED, "mesh nvar must be 5");
+    if (m->ncoarse != m->nx * m->ny * m->nz || m->ncell != m->ncoarse + 8 * m->ngridmax)
+        return fail(c, RH_ERR_ARG, "ncoarse/ncell inconsistent with nx,ny,nz,ngridmax");
+    if (m->ncpu < 1 || m->myid < 1 || m->myid > m->ncpu + m->nboundary || m->nlevelmax < 1)
+        return fail(c, RH_ERR_ARG, "bad ncpu/myid/nlevelmax");
+    if (!m->father || !m->son || !m->nbor || !m->next || !m->headl || !m->uold || !m->unew)
+        return fail(c, RH_ERR_ARG, "null mesh array");
+    RH_HIP(c, hipSetDevice(c->p.device));
+    const size_t n = (size_t)m->nvar * m->ncell;
+    bool topo_changed = !c->bound || c->mesh.mesh_epoch != m->mesh_epoch || c->mesh.father != m->father ||
+                        c->mesh.son != m->son || c->mesh.nbor != m->nbor || c->mesh.next != m->next ||
+                        c->mesh.headl != m->headl || c->mesh.ncell != m->ncell;
+    if (n > c->state_cap) {
+        if (c->d_uold) (void)hipFree(c->d_uold);
+        if (c->d_unew) (void)hipFree(c->d_unew);
+        c->d_uold = c->d_unew = nullptr;
+        c->state_cap = 0;
+        RH_HIP(c, hipMalloc(&c->d_uold, n * sizeof(double)));
+        RH_HIP(c, hipMalloc(&c->d_unew, n * sizeof(double)));
+        c->state_cap = n;
+    }
+    c->mesh = *m;
+    c->mode = mode;
+    c->bound = true;
+    if (topo_changed) {
+        c->topo_valid = false;
+        for (auto& L : c->levels) free_level(L);
+    }
+    int st = upload_state(c, RH_UOLD | RH_UNEW);
+    if (st) return st;
+    if (m->f) {
+        size_t nf = (size_t)3 * m->ncell;
+        if (nf > c->f_cap) {
+            if (c->d_f) (void)hipFree(c->d_f);
+            c->d_f = nullptr;
+            RH_HIP(c, hipMalloc(&c->d_f, nf * sizeof(double)));
+            c->f_cap = nf;
+        }
+        RH_HIP(c, hipMemcpyAsync(c->d_f, m->f, nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    } else if (c->d_f) {
+        (void)hipFree(c->d_f);
+        c->d_f = nullptr;
+        c->f_cap = 0;
+    }
+    RH_HIP(c, hipStreamSynchronize(c->stream));
+    return RH_OK;
+}
+
+int rh_set_unew(rh_ctx* c, int ilevel) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    if ((st = pre_call(c))) return st;
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    Timed tm(c, TK_SET_UNEW);
+    RH_HIP(c, rh::launch_set_unew(mesh_dev(c), *lv, c->stream));
+    tm.done();
+    return post_call(c, RH_UNEW);
+}
+
+int rh_set_uold(rh_ctx* c, int ilevel) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    if ((st = pre_call(c))) return st;
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    Timed tm(c, TK_SET_UOLD);
+    RH_HIP(c, rh::launch_set_uold(mesh_dev(c), *lv, c->stream));
+    tm.done();
+    return post_call(c, RH_UOLD);
+}
+
+int rh_godunov_fine(rh_ctx* c, int ilevel, double dt, double dx) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    if ((st = pre_call(c))) return st;
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    if ((st = ensure_dt(c, 1))) return st;
+    RH_HIP(c, hipMemcpyAsync(c->d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((st = do_godunov(c, lv, c->d_dt, dx))) return st;
+    RH_HIP(c, hipStreamSynchronize(c->stream));  // dt staging value is on the stack
+    return post_call(c, RH_UNEW);
+}
+
+int rh_courant(rh_ctx* c, int ilevel, double dx, double gamma, double courant_factor, double* dt_out) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (!dt_out) return fail(c, RH_ERR_ARG, "null dt_out");
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    if ((st = pre_call(c))) return st;
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    if ((st = ensure_dt(c, 1))) return st;
+    if ((st = do_courant(c, lv, dx, gamma, courant_factor, c->d_dt))) return st;
+    RH_HIP(c, hipMemcpyAsync(dt_out, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    RH_HIP(c, hipStreamSynchronize(c->stream));
+    return RH_OK;
+}
+
+int rh_interpol_hydro(rh_ctx* c, int n_octs, const double* u1, double* u2) {
+    if (!c) return RH_ERR_ARG;
+    if (n_octs < 0 || (n_octs > 0 && (!u1 || !u2))) return fail(c, RH_ERR_ARG, "bad interpol arguments");
+    if (n_octs == 0) return RH_OK;
+    RH_HIP(c, hipSetDevice(c->p.device));
+    if (n_octs > c->interpol_cap) {
+        if (c->d_u1) (void)hipFree(c->d_u1);
+        if (c->d_u2) (void)hipFree(c->d_u2);
+        c->d_u1 = c->d_u2 = nullptr;
+        c->interpol_cap = 0;
+        RH_HIP(c, hipMalloc(&c->d_u1, (size_t)n_octs * 7 * 64 * sizeof(double)));
+        RH_HIP(c, hipMalloc(&c->d_u2, (size_t)n_octs * 8 * 64 * sizeof(double)));
+        c->interpol_cap = n_octs;
+    }
+    RH_HIP(c, hipMemcpyAsync(c->d_u1, u1, (size_t)n_octs * 7 * 64 * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    // Rows beyond nvar are returned unchanged from what the caller passed.
+    RH_HIP(c, hipMemcpyAsync(c->d_u2, u2, (size_t)n_octs * 8 * 64 * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    RH_HIP(c, rh::launch_interpol(n_octs, c->d_u1, c->d_u2, gas(c, c->p.gamma), c->p.interpol_var, c->stream));
+    RH_HIP(c, hipMemcpyAsync(u2, c->d_u2, (size_t)n_octs * 8 * 64 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    RH_HIP(c, hipStreamSynchronize(c->stream));
+    return RH_OK;
+}
+
+int rh_sync(rh_ctx* c, int direction, int what) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (direction == RH_SYNC_TO_HOST) return download_state(c, what);
+    if (direction == RH_SYNC_TO_DEVICE) {
+        if ((st = upload_state(c, what))) return st;
+        RH_HIP(c, hipStreamSynchronize(c->stream));
+        return RH_OK;
+    }
+    return fail(c, RH_ERR_ARG, "unknown sync direction");
+}
+
+const char* rh_last_error(const rh_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int nsteps, double* dt_out) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (nsteps < 0) return fail(c, RH_ERR_ARG, "nsteps < 0");
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    if ((st = pre_call(c))) return st;
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    if ((st = ensure_dt(c, nsteps > 0 ? nsteps : 1))) return st;
+    const rh::MeshDev m = mesh_dev(c);
+    for (int k = 0; k < nsteps; ++k) {
+        if ((st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt + k))) return st;
+        {
+            Timed tm(c, TK_SET_UNEW);
+            RH_HIP(c, rh::launch_set_unew(m, *lv, c->stream));
+            tm.done();
+        }
+        if ((st = do_godunov(c, lv, c->d_dt + k, dx))) return st;
+        {
+            Timed tm(c, TK_SET_UOLD);
+            RH_HIP(c, rh::launch_set_uold(m, *lv, c->stream));
+            tm.done();
+        }
+    }
+    if (dt_out && nsteps > 0)
+        RH_HIP(c, hipMemcpyAsync(dt_out, c->d_dt, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    RH_HIP(c, hipStreamSynchronize(c->stream));
+    return post_call(c, RH_UOLD | RH_UNEW);
+}
+
+int rh_timers_enable(rh_ctx* c, int enable) {
+    if (!c) return RH_ERR_ARG;
+    c->timers = enable != 0;
+    return RH_OK;
+}
+
+int rh_timers_read(rh_ctx* c, int kernel, double* ms_total, long long* count) {
+    if (!c || kernel < 0 || kernel >= TK_N) return RH_ERR_ARG;
+    collect_timers(c);
+    if (ms_total) *ms_total = c->t_ms[kernel];
+    if (count) *count = c->t_n[kernel];
+    return RH_OK;
+}
+
+int rh_timers_reset(rh_ctx* c) {
+    if (!c) return RH_ERR_ARG;
+    collect_timers(c);
+    for (int k = 0; k < TK_N; ++k) {
+        c->t_ms[k] = 0;
+        c->t_n[k] = 0;
+    }
+    return RH_OK;
+}
+
+int rh_level_leaf_cells(rh_ctx* c, int ilevel, long long* nleaf) {
+    int st = check_bound(c);
+    if (st) return st;
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    if (nleaf) *nleaf = lv->nleaf;
+    return RH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// rh_kernels.h -- device-side layout of one level's tables and the kernel
+// launch helpers shared by the C-ABI implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace rh {
+
+// Record flags (same values as the host tables in rh_tables.h).
+constexpr int REC_REFINED_DEV = 1;
+constexpr int REC_BOUNDARY_DEV = 2;
+constexpr int REC_REFLUX_DEV = 4;
+
+// Device copy of one level's tables (rh_tables.h describes the contents).
+struct LevelDev {
+    int noct = 0;
+    int nrec = 0;
+    int nslots = 0;
+    int ntgt = 0;
+    long long nleaf = 0;
+    int* grid = nullptr;        // [noct]
+    uint8_t* leaf = nullptr;    // [noct]
+    int* nbo = nullptr;         // [6*noct]
+    int4* rec = nullptr;        // [nrec] {n, nl, nr, flags}
+    int* rec_slot = nullptr;    // [nrec]
+    int* tgt_cell = nullptr;    // [ntgt]
+    int* tgt_off = nullptr;     // [ntgt+1]
+    int* tgt_slot = nullptr;    // [nslots used]
+    double* slot_r = nullptr;   // [5*nslots] coarse reflux contributions
+};
+
+// Geometry / state of the bound mesh on the device.
+struct MeshDev {
+    int ncoarse, ngridmax, ncell;
+    int nx, ny, nz;
+    double* uold;   // [5*ncell]
+    double* unew;   // [5*ncell]
+    const double* f;  // [3*ncell] or null
+};
+
+struct GasParams {
+    double gamma, gm1, smalle, smallp;
+    int slope_type, interpol_type;
+};
+
+// Launchers (defined in rh_hydro.hip).  All enqueue on `stream`.
+hipError_t launch_set_unew(const MeshDev& m, const LevelDev& lv, hipStream_t stream);
+hipError_t launch_set_uold(const MeshDev& m, const LevelDev& lv, hipStream_t stream);
+// Courant minimum into *d_dt (device).  d_part has >= courant_blocks(lv) doubles.
+hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
+                          double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream);
+int courant_blocks(const LevelDev& lv);
+// Godunov update of one level; dt is read from device memory (*d_dt).
+hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams& gp, int riemann, int precision,
+                          const double* d_dt, double dx, hipStream_t stream);
+hipError_t launch_interpol(int n_octs, const double* d_u1, double* d_u2, const GasParams& gp, int interpol_var,
+                           hipStream_t stream);
+
+}  // namespace rh

@@ -1,0 +1,419 @@
+// rh_physics.h -- per-cell physics of the Godunov hydro path, as device
+// functions.  Every routine is templated on the arithmetic type T: T=double
+// is the bit-faithful FP64 form, T=float the compute half of the mixed form.
+//
+// Bit-faithfulness rules for T=double (must match the reference compiled by
+// g++ for x86-64 without FMA):
+//   * the whole library is compiled with -ffp-contract=off, no fast-math;
+//   * every expression keeps the reference's left-to-right association;
+//   * std::max(a,b) is (a<b)?b:a, std::min(a,b) is (b<a)?b:a,
+//     std::clamp(v,lo,hi) is (v<lo)?lo:((hi<v)?hi:v), std::min({a,b,c}) keeps
+//     the first smallest element -- written out as ternaries so NaN and -0
+//     behave as in libstdc++;
+//   * '/' and sqrt are the correctly rounded IEEE operations.
+// State vectors are 5 components: rho, 3 velocity (or momentum), p (or E).
+// Rotated frames put the normal velocity at index 1 by swapping 1<->1+idim.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define RH_DEV __device__ __forceinline__
+
+namespace rh {
+
+template <typename T> RH_DEV T smax(T a, T b) { return (a < b) ? b : a; }
+template <typename T> RH_DEV T smin(T a, T b) { return (b < a) ? b : a; }
+template <typename T> RH_DEV T sclamp(T v, T lo, T hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+template <typename T> RH_DEV T smin3(T a, T b, T c) {
+    T m = a;
+    m = (b < m) ? b : m;
+    m = (c < m) ? c : m;
+    return m;
+}
+template <typename T> RH_DEV T sabs(T a) { return __builtin_fabs(a); }
+RH_DEV double ssqrt(double a) { return __builtin_sqrt(a); }
+RH_DEV float ssqrt(float a) { return __builtin_sqrtf(a); }
+
+template <typename T> RH_DEV void swap_(T& a, T& b) {
+    T t = a;
+    a = b;
+    b = t;
+}
+
+// Rotate a 5-vector into the frame of dimension d (swap 1 <-> 1+d).
+template <typename T> RH_DEV void rotate(T q[5], int d) {
+    if (d == 1) swap_(q[1], q[2]);
+    else if (d == 2) swap_(q[1], q[3]);
+}
+
+// Per-call constants derived from the namelist.
+template <typename T> struct Gas {
+    T gamma;     // ratio of specific heats
+    T gm1;       // gamma - 1
+    T smalle;    // ((1e-10*1e-10)/gamma)/(gamma-1), ctoprim floor
+    T smallp;    // (1e-10*1e-10)/gamma, Riemann pressure floor factor
+};
+
+// Conservative -> primitive (HydroSolver.cpp:613-651, nener=0).
+template <typename T> RH_DEV void ctoprim(const T u[5], T q[5], const Gas<T>& g) {
+    T d = smax(u[0], T(1e-10));
+    q[0] = d;
+    T oneoverrho = T(1.0) / d;
+    T eken = T(0.0);
+    q[1] = u[1] * oneoverrho;
+    eken = eken + T(0.5) * q[1] * q[1];
+    q[2] = u[2] * oneoverrho;
+    eken = eken + T(0.5) * q[2] * q[2];
+    q[3] = u[3] * oneoverrho;
+    eken = eken + T(0.5) * q[3] * q[3];
+    T erad = T(0.0);
+    T eint = smax(u[4] * oneoverrho - eken - erad, g.smalle);
+    q[4] = g.gm1 * d * eint;
+}
+
+// Sound speed squared (EquationOfState.cpp:26-51, no barotropic EOS, nener=0).
+template <typename T> RH_DEV T cs2(T density, T pressure, const Gas<T>& g) {
+    T c = g.gamma * pressure;
+    return smax(c / smax(density, T(1e-10)), T(1e-20));
+}
+
+// TVD slope limiter (SlopeLimiter.cpp:7-60), theta = 1.5.
+template <typename T> RH_DEV T limiter(T ql, T qc, T qr, int slope_type, T nu) {
+    if (slope_type == 0) return T(0.0);
+    T dlft = qc - ql;
+    T drgt = qr - qc;
+    if (slope_type == 4 || slope_type == 5) {
+        T dlft_s, drgt_s;
+        if (slope_type == 4) {
+            dlft_s = T(2.0) / (T(1.0) + nu) * dlft;
+            drgt_s = T(2.0) / (T(1.0) - nu) * drgt;
+        } else if (nu >= T(0.0)) {
+            dlft_s = T(2.0) / (nu + T(1e-10)) * dlft;
+            drgt_s = T(2.0) / (T(1.0) - nu) * drgt;
+        } else {
+            dlft_s = T(2.0) / (T(1.0) + nu) * dlft;
+            drgt_s = T(2.0) / (-nu + T(1e-10)) * drgt;
+        }
+        T dcen = T(0.5) * (dlft + drgt);
+        T dsgn = (dcen >= T(0.0)) ? T(1.0) : T(-1.0);
+        T dlim = smin(sabs(dlft_s), sabs(drgt_s));
+        if (dlft * drgt <= T(0.0)) dlim = T(0.0);
+        return dsgn * dlim;
+    }
+    if (slope_type == 6) return T(0.5) * (dlft + drgt);
+    if (dlft * drgt <= T(0.0)) return T(0.0);
+    T dcen = T(0.5) * (dlft + drgt);
+    T dsgn = (dcen >= T(0.0)) ? T(1.0) : T(-1.0);
+    T dlim;
+    if (slope_type == 1) {
+        dlim = smin(sabs(dlft), sabs(drgt));
+    } else if (slope_type == 2 || slope_type == 3) {
+        dlim = smin3(T(2.0) * sabs(dlft), T(2.0) * sabs(drgt), sabs(dcen));
+    } else if (slope_type == 7) {
+        dlim = T(2.0) * sabs(dlft) * sabs(drgt) / (sabs(dlft) + sabs(drgt));
+    } else if (slope_type == 8) {
+        dlim = smin3(T(1.5) * sabs(dlft), T(1.5) * sabs(drgt), sabs(dcen));
+    } else {
+        dlim = smin(sabs(dlft), sabs(drgt));
+    }
+    return dsgn * dlim;
+}
+
+// Interpolation limiter of the coarse-ghost state (HydroSolver.cpp:391-404).
+template <typename T> RH_DEV T ghost_limiter(T dlft, T drgt, int interpol_type) {
+    T s = T(0.0);
+    if (interpol_type > 0 && dlft * drgt > T(0.0)) {
+        T sgn = (dlft >= T(0.0)) ? T(1.0) : T(-1.0);
+        if (interpol_type == 1) s = sgn * T(0.5) * smin(sabs(dlft), sabs(drgt));
+        else if (interpol_type == 2) s = sgn * smin3(sabs(dlft), sabs(drgt), T(0.25) * sabs(dlft + drgt));
+        else if (interpol_type == 3) s = T(0.25) * (dlft + drgt);
+    }
+    return s;
+}
+
+// MUSCL-Hancock predictor in a rotated frame (HydroSolver.cpp:680-732).
+template <typename T>
+RH_DEV void trace(const T q[5], const T dq[5], T dt, T dx, T qm[5], T qp[5], const Gas<T>& g) {
+    T r = smax(q[0], T(1e-10));
+    T u = q[1];
+    T p = smax(q[4], r * T(1e-20));
+    T sr0 = -u * dq[0] - dq[1] * r;
+    T su0 = -u * dq[1] - dq[4] / r;
+    T c2 = cs2(r, p, g);
+    T sp0 = -u * dq[4] - dq[1] * r * c2;
+    T hdx = T(0.5) * dx;
+    T hdt = T(0.5) * dt;
+    T src[5] = {sr0, su0, T(0), T(0), sp0};
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) {
+        qp[iv] = q[iv] - hdx * dq[iv] + hdt * src[iv];
+        qm[iv] = q[iv] + hdx * dq[iv] + hdt * src[iv];
+    }
+    if (qp[0] < T(1e-10)) qp[0] = r;
+    if (qm[0] < T(1e-10)) qm[0] = r;
+}
+
+// Own face states of a cell along dimension d (HydroSolver.cpp:93-115 with
+// compute_slopes :653-678).  ql/qc/qr are the unrotated primitives of the
+// left neighbour, the cell and the right neighbour; outputs are rotated.
+template <typename T>
+RH_DEV void own_trace(const T ql[5], const T qc[5], const T qr[5], int d, int slope_type, T dt, T dx, T qm[5],
+                      T qp[5], const Gas<T>& g) {
+    T a[5], b[5], c[5];
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) {
+        a[iv] = ql[iv];
+        b[iv] = qc[iv];
+        c[iv] = qr[iv];
+    }
+    rotate(a, d);
+    rotate(b, d);
+    rotate(c, d);
+    T dq[5];
+    if (slope_type == 0) {
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) dq[iv] = T(0.0);
+    } else {
+        T nu = b[1] * dt / dx;
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            if ((slope_type == 5 || slope_type == 6) && iv != 0) dq[iv] = T(0.0);
+            else dq[iv] = limiter(a[iv], b[iv], c[iv], slope_type, nu) / dx;
+        }
+    }
+    trace(b, dq, dt, dx, qm, qp, g);
+}
+
+// Ghost face states of a neighbour cell n along dimension d, interpolated
+// with interpol_type slopes over 2*dx (HydroSolver.cpp:357-440).  Inputs are
+// unrotated primitives of n's left neighbour, n, n's right neighbour.
+template <typename T>
+RH_DEV void ghost_trace(const T ql[5], const T qc[5], const T qr[5], int d, int interpol_type, T dt, T dx,
+                        T qm[5], T qp[5], const Gas<T>& g) {
+    T a[5], b[5], c[5];
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) {
+        a[iv] = ql[iv];
+        b[iv] = qc[iv];
+        c[iv] = qr[iv];
+    }
+    rotate(a, d);
+    rotate(b, d);
+    rotate(c, d);
+    T dx_coarse = T(2.0) * dx;
+    T dq[5];
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) dq[iv] = ghost_limiter(b[iv] - a[iv], c[iv] - b[iv], interpol_type) / dx_coarse;
+    trace(b, dq, dt, dx_coarse, qm, qp, g);
+}
+
+// Ghost of a boundary face: the cell's own state, normal momentum flipped for
+// reflective boundaries, traced with zero slopes (HydroSolver.cpp:130-145).
+template <typename T>
+RH_DEV void boundary_trace(const T u_own[5], bool reflect, int d, T dt, T dx, T qm[5], T qp[5], const Gas<T>& g) {
+    T u[5];
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) u[iv] = u_own[iv];
+    if (reflect) u[1 + d] = u[1 + d] * T(-1.0);
+    T q[5];
+    ctoprim(u, q, g);
+    rotate(q, d);
+    T dq[5] = {T(0), T(0), T(0), T(0), T(0)};
+    trace(q, dq, dt, dx, qm, qp, g);
+}
+
+// RiemannSolver.cpp:385-402
+template <typename T> RH_DEV void prim_to_cons(const T q[5], T u[5], const Gas<T>& g) {
+    u[0] = q[0];
+    T v2 = T(0);
+    u[1] = q[0] * q[1];
+    v2 = v2 + q[1] * q[1];
+    u[2] = q[0] * q[2];
+    v2 = v2 + q[2] * q[2];
+    u[3] = q[0] * q[3];
+    v2 = v2 + q[3] * q[3];
+    T e_kin = T(0.5) * q[0] * v2;
+    T e_int = q[4] / g.gm1;
+    T e_nonthermal = T(0.0);
+    u[4] = e_kin + e_int + e_nonthermal;
+}
+
+// RiemannSolver.cpp:404-427
+template <typename T> RH_DEV void phys_flux(const T q[5], T f[5], const Gas<T>& g) {
+    T rho = q[0];
+    T u = q[1];
+    T p = q[4];
+    T v2 = T(0);
+    v2 = v2 + q[1] * q[1];
+    v2 = v2 + q[2] * q[2];
+    v2 = v2 + q[3] * q[3];
+    T e_kin = T(0.5) * rho * v2;
+    T e_int = p / g.gm1;
+    T e_nonthermal = T(0.0);
+    T p_tot = p;
+    T E = e_kin + e_int + e_nonthermal;
+    f[0] = rho * u;
+    f[1] = rho * u * u + p_tot;
+    f[2] = rho * u * q[2];
+    f[3] = rho * u * q[3];
+    f[4] = (E + p_tot) * u;
+}
+
+// RiemannSolver.cpp:15-30
+template <typename T> RH_DEV void riemann_llf(const T ql[5], const T qr[5], T flux[5], const Gas<T>& g) {
+    T fl[5], fr[5], ul[5], ur[5];
+    phys_flux(ql, fl, g);
+    phys_flux(qr, fr, g);
+    prim_to_cons(ql, ul, g);
+    prim_to_cons(qr, ur, g);
+    T al = ssqrt(cs2(ql[0], ql[4], g));
+    T ar = ssqrt(cs2(qr[0], qr[4], g));
+    T a_max = smax(sabs(ql[1]) + al, sabs(qr[1]) + ar);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) flux[i] = T(0.5) * (fl[i] + fr[i]) - T(0.5) * a_max * (ur[i] - ul[i]);
+}
+
+// RiemannSolver.cpp:32-66
+template <typename T> RH_DEV void riemann_hll(const T ql[5], const T qr[5], T flux[5], const Gas<T>& g) {
+    T fl[5], fr[5], ul[5], ur[5];
+    phys_flux(ql, fl, g);
+    phys_flux(qr, fr, g);
+    prim_to_cons(ql, ul, g);
+    prim_to_cons(qr, ur, g);
+    T al = ssqrt(cs2(ql[0], ql[4], g));
+    T ar = ssqrt(cs2(qr[0], qr[4], g));
+    T sl = smin(T(0.0), smin(ql[1] - al, qr[1] - ar));
+    T sr = smax(T(0.0), smax(ql[1] + al, qr[1] + ar));
+    if (sr - sl < T(1e-20)) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) flux[i] = T(0.5) * (fl[i] + fr[i]);
+    } else {
+        T inv_srs = T(1.0) / (sr - sl);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) flux[i] = (sr * fl[i] - sl * fr[i] + sr * sl * (ur[i] - ul[i])) * inv_srs;
+    }
+}
+
+// RiemannSolver.cpp:68-192 (nener=0)
+template <typename T> RH_DEV void riemann_hllc(const T ql[5], const T qr[5], T flux[5], const Gas<T>& g) {
+    const T smallr = T(1e-10);
+    T rl = smax(ql[0], smallr);
+    T ul = ql[1];
+    T pl = smax(ql[4], rl * g.smallp);
+    T rr = smax(qr[0], smallr);
+    T ur = qr[1];
+    T pr = smax(qr[4], rr * g.smallp);
+    T cl = ssqrt(cs2(rl, ql[4], g));
+    T cr = ssqrt(cs2(rr, qr[4], g));
+    T cmax = smax(cl, cr);
+    T sl = smin(ul, ur) - cmax;
+    T sr = smax(ul, ur) + cmax;
+    if (sl > T(0.0)) {
+        phys_flux(ql, flux, g);
+        return;
+    }
+    if (sr < T(0.0)) {
+        phys_flux(qr, flux, g);
+        return;
+    }
+    T rcl = rl * (ul - sl);
+    T rcr = rr * (sr - ur);
+    T div = rcr + rcl;
+    if (div < T(1e-5)) {
+        T fl[5], fr[5];
+        phys_flux(ql, fl, g);
+        phys_flux(qr, fr, g);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) flux[i] = T(0.5) * (fl[i] + fr[i]);
+        return;
+    }
+    T ustar = (rcr * ur + rcl * ul + (pl - pr)) / div;
+    ustar = smax(T(-1e6), smin(T(1e6), ustar));
+    T pstar = (rcr * pl + rcl * pr + rcl * rcr * (ul - ur)) / div;
+    pstar = smax(pstar, T(0.0));
+    // One side of the contact: the left star state for ustar > 0, otherwise
+    // the right one; both branches share this code with mirrored inputs.
+    const bool left = ustar > T(0.0);
+    const T* qs_src = left ? ql : qr;
+    T rk = left ? rl : rr;
+    T uk = left ? ul : ur;
+    T pk = left ? pl : pr;
+    T sk = left ? sl : sr;
+    T den = left ? smin(sl - ustar, T(-1e-10)) : smax(sr - ustar, T(1e-10));
+    T rstar = rk * (sk - uk) / den;
+    T qs[5];
+    qs[0] = rstar;
+    qs[1] = ustar;
+    qs[2] = qs_src[2];
+    qs[3] = qs_src[3];
+    T sum_prad_star = T(0.0);
+    qs[4] = smax(pstar - sum_prad_star, rstar * g.smallp);
+    T fk[5], uk_v[5], us_v[5];
+    phys_flux(qs_src, fk, g);
+    prim_to_cons(qs_src, uk_v, g);
+    prim_to_cons(qs, us_v, g);
+    us_v[4] = ((sk - uk) * uk_v[4] - pk * uk + pstar * ustar) / den;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) flux[i] = fk[i] + sk * (us_v[i] - uk_v[i]);
+}
+
+// RiemannSolver.cpp:309-383 (nener=0)
+template <typename T> RH_DEV void riemann_acoustic(const T ql[5], const T qr[5], T flux[5], const Gas<T>& g) {
+    const T smallr = T(1e-10);
+    const T smallc = T(1e-10);
+    T rl = smax(ql[0], smallr);
+    T ul = ql[1];
+    T pl = smax(ql[4], rl * g.smallp);
+    T rr = smax(qr[0], smallr);
+    T ur = qr[1];
+    T pr = smax(qr[4], rr * g.smallp);
+    T cl = ssqrt(cs2(rl, ql[4], g));
+    T cr = ssqrt(cs2(rr, qr[4], g));
+    T wl = cl * rl;
+    T wr = cr * rr;
+    T pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr + T(1e-20));
+    T ustar = ((wr * ur + wl * ul) + (pl - pr)) / (wl + wr + T(1e-20));
+    T sgnm = (ustar >= T(0.0)) ? T(1.0) : T(-1.0);
+    bool lft = sgnm == T(1.0);
+    T ro = lft ? rl : rr, uo = lft ? ul : ur, po = lft ? pl : pr, co = lft ? cl : cr;
+    T rstar = ro + (pstar - po) / smax(co * co, T(1e-20));
+    rstar = smax(rstar, smallr);
+    T cstar = ssqrt(smax(g.gamma * pstar / rstar, T(1e-20)));
+    cstar = smax(cstar, smallc);
+    T spout = co - sgnm * uo;
+    T spin = cstar - sgnm * ustar;
+    T ushock = T(0.5) * (spin + spout);
+    ushock = smax(ushock, -sgnm * ustar);
+    if (pstar >= po) {
+        spout = ushock;
+        spin = spout;
+    }
+    T rg, ug, pg;
+    if (spout < T(0.0)) {
+        rg = ro; ug = uo; pg = po;
+    } else if (spin >= T(0.0)) {
+        rg = rstar; ug = ustar; pg = pstar;
+    } else {
+        T frac = spout / (spout - spin + T(1e-20));
+        rg = frac * rstar + (T(1.0) - frac) * ro;
+        ug = frac * ustar + (T(1.0) - frac) * uo;
+        pg = frac * pstar + (T(1.0) - frac) * po;
+    }
+    T qg[5];
+    qg[0] = rg;
+    qg[1] = ug;
+    qg[2] = lft ? ql[2] : qr[2];
+    qg[3] = lft ? ql[3] : qr[3];
+    T sum_prad_star = T(0.0);
+    qg[4] = smax(pg - sum_prad_star, rg * g.smallp);
+    phys_flux(qg, flux, g);
+}
+
+template <int RIEMANN, typename T>
+RH_DEV void riemann(const T ql[5], const T qr[5], T flux[5], const Gas<T>& g) {
+    if constexpr (RIEMANN == 2) riemann_hllc(ql, qr, flux, g);
+    else if constexpr (RIEMANN == 1) riemann_hll(ql, qr, flux, g);
+    else if constexpr (RIEMANN == 4) riemann_acoustic(ql, qr, flux, g);
+    else riemann_llf(ql, qr, flux, g);
+}
+
+}  // namespace rh

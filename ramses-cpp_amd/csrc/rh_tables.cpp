@@ -1,0 +1,267 @@
+// rh_tables.cpp -- host construction of the per-level index tables (see
+// rh_tables.h).  Index preprocessing only; no state arithmetic happens here.
+#include "rh_tables.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <numeric>
+
+namespace rh {
+
+namespace {
+
+// Octant geometry of the reference (Constants.cpp:8-43): the neighbour of
+// cell ic (0-based ic0 = ix + 2*iy + 4*iz) across dimension d lies in the
+// same oct when bit d of ic0 differs from the side (0 = minus, 1 = plus),
+// otherwise in the neighbouring oct in direction 2*d+side, and in both cases
+// at cell ic0 ^ (1<<d).
+inline bool crosses(int ic0, int d, int side) { return ((ic0 >> d) & 1) == side; }
+
+struct MeshView {
+    const rh_mesh& m;
+    int headl(int il) const {
+        if (il < 0 || il > m.nlevelmax) return 0;
+        return m.headl[(size_t)il * (m.ncpu + m.nboundary) + (m.myid - 1)];
+    }
+    int nbor(int dir, int igrid) const {
+        int v = m.nbor[(size_t)dir * m.ngridmax + igrid - 1];
+        return (v > m.ncell) ? 0 : v;  // AmrGrid.cpp:394-396
+    }
+    int cell(int igrid, int ic0) const { return m.ncoarse + ic0 * m.ngridmax + igrid; }
+    int grid_of(int icell) const { return ((icell - m.ncoarse - 1) % m.ngridmax) + 1; }
+    int ic0_of(int icell) const { return (icell - m.ncoarse - 1) / m.ngridmax; }
+};
+
+// AmrGrid.cpp:513-550 with no boundary regions: periodic wrap.
+void coarse_nbors(const rh_mesh& m, int icell, int icn[6]) {
+    for (int i = 0; i < 6; ++i) icn[i] = 0;
+    if (icell <= 0 || icell > m.ncoarse) return;
+    int idx = icell - 1;
+    int nx = m.nx, ny = m.ny, nz = m.nz;
+    int x[3] = {idx % nx, (idx % (nx * ny)) / nx, idx / (nx * ny)};
+    int n[3] = {nx, ny, nz};
+    for (int d = 0; d < 3; ++d)
+        for (int s = 0; s < 2; ++s) {
+            int c[3] = {x[0], x[1], x[2]};
+            c[d] = (c[d] + (s == 0 ? -1 : 1) + n[d]) % n[d];
+            icn[2 * d + s] = c[2] * nx * ny + c[1] * nx + c[0] + 1;
+        }
+}
+
+// AmrGrid.cpp:402-444 for cell (igrid, ic0).
+void oct_cell_nbors(const rh_mesh& m, int igrid, int ic0, int icn[6]) {
+    MeshView v{m};
+    for (int d = 0; d < 3; ++d)
+        for (int s = 0; s < 2; ++s) {
+            int f = 2 * d + s;
+            int ic_pos = ic0 ^ (1 << d);
+            if (!crosses(ic0, d, s)) {
+                icn[f] = v.cell(igrid, ic_pos);
+                continue;
+            }
+            int ig = v.nbor(f, igrid);
+            if (ig > 0) {
+                int child = (ig <= m.ncell) ? m.son[ig - 1] : 0;
+                icn[f] = child > 0 ? v.cell(child, ic_pos) : ig;
+            } else if (ig < 0) {
+                icn[f] = ig;
+            } else {
+                int ifather = m.father[igrid - 1];
+                int fn[6];
+                nbor_cells_of_cell(m, ifather, fn);
+                icn[f] = fn[f];
+            }
+        }
+}
+
+}  // namespace
+
+void nbor_cells_of_cell(const rh_mesh& m, int icell, int icn[6]) {
+    if (icell <= m.ncoarse) {
+        coarse_nbors(m, icell, icn);
+        return;
+    }
+    MeshView v{m};
+    oct_cell_nbors(m, v.grid_of(icell), v.ic0_of(icell), icn);
+}
+
+int build_topo_index(const rh_mesh& m, TopoIndex& ti, std::string& err) {
+    MeshView v{m};
+    ti.glev.assign((size_t)m.ngridmax + 1, 0);
+    ti.gpos.assign((size_t)m.ngridmax + 1, -1);
+    ti.depth.assign((size_t)m.ngridmax + 1, 0);
+    for (int il = 1; il <= m.nlevelmax; ++il) {
+        int ig = v.headl(il);
+        int pos = 0;
+        while (ig > 0) {
+            if (ig > m.ngridmax) {
+                err = "level list " + std::to_string(il) + " holds grid index beyond ngridmax";
+                return RH_ERR_TOPOLOGY;
+            }
+            if (ti.glev[ig] != 0) {
+                err = "grid " + std::to_string(ig) + " appears in two level lists (or a list is cyclic)";
+                return RH_ERR_TOPOLOGY;
+            }
+            ti.glev[ig] = il;
+            ti.gpos[ig] = pos++;
+            ig = m.next[ig - 1];
+        }
+    }
+    // Father-chain depth, as HydroSolver.cpp:22-38 computes cell_levels.
+    for (int ig = 1; ig <= m.ngridmax; ++ig) {
+        if (m.father[ig - 1] <= 0) continue;
+        int level = 1;
+        int cur = ig;
+        int guard = 0;
+        while (cur > 0) {
+            int fc = m.father[cur - 1];
+            if (fc <= m.ncoarse) break;
+            cur = ((fc - m.ncoarse - 1) % m.ngridmax) + 1;
+            level++;
+            if (++guard > 64) {
+                err = "father chain longer than 64 from grid " + std::to_string(ig);
+                return RH_ERR_TOPOLOGY;
+            }
+        }
+        ti.depth[ig] = level;
+    }
+    ti.epoch = m.mesh_epoch;
+    return RH_OK;
+}
+
+int build_level(const rh_mesh& m, const TopoIndex& ti, int level, LevelHost& out, std::string& err) {
+    MeshView v{m};
+    out = LevelHost{};
+    out.level = level;
+    auto cell_level = [&](int icell) -> int {  // cell_levels[icell] of the reference
+        if (icell <= m.ncoarse) return 0;
+        return ti.depth[v.grid_of(icell)];
+    };
+    // Oct list in processing order.
+    for (int ig = v.headl(level); ig > 0; ig = m.next[ig - 1]) out.grid.push_back(ig);
+    const int noct = (int)out.grid.size();
+    out.noct = noct;
+    out.leaf.assign(noct, 0);
+    out.nbo.assign((size_t)6 * noct, 0);
+    for (int p = 0; p < noct; ++p) {
+        int g = out.grid[p];
+        if (ti.depth[g] != level - 1) {
+            err = "oct " + std::to_string(g) + " of level list " + std::to_string(level) + " has depth " +
+                  std::to_string(ti.depth[g]) + " (expected level-1)";
+            return RH_ERR_TOPOLOGY;
+        }
+        uint8_t lm = 0;
+        for (int ic0 = 0; ic0 < 8; ++ic0)
+            if (m.son[v.cell(g, ic0) - 1] == 0) lm |= (uint8_t)(1u << ic0);
+        out.leaf[p] = lm;
+        out.nleaf += __builtin_popcount(lm);
+    }
+    // Neighbour octs / records.
+    for (int p = 0; p < noct; ++p) {
+        int g = out.grid[p];
+        for (int f = 0; f < 6; ++f) {
+            int d = f >> 1;
+            int nb = v.nbor(f, g);
+            int n = 0;
+            if (nb > 0) {
+                int s = m.son[nb - 1];
+                if (s > 0) {
+                    if (ti.glev[s] != level) {
+                        err = "neighbour oct " + std::to_string(s) + " of oct " + std::to_string(g) +
+                              " is not in level list " + std::to_string(level);
+                        return RH_ERR_TOPOLOGY;
+                    }
+                    out.nbo[(size_t)f * noct + p] = ti.gpos[s];
+                    continue;
+                }
+                n = nb;
+            } else if (nb < 0) {
+                n = nb;
+            } else {
+                int fn[6];
+                nbor_cells_of_cell(m, m.father[g - 1], fn);
+                n = fn[f];
+            }
+            int r = (int)out.rec_n.size();
+            int flags = 0, nl = 0, nr = 0;
+            if (n <= 0) {
+                flags = REC_BOUNDARY;
+            } else {
+                bool refined = m.son[n - 1] > 0 || cell_level(n) > level;
+                if (refined) {
+                    flags = REC_REFINED;
+                } else {
+                    if (cell_level(n) == level) {
+                        err = "face neighbour " + std::to_string(n) + " of oct " + std::to_string(g) +
+                              " reaches the reference's same-level branch (HydroSolver.cpp:148-161)";
+                        return RH_ERR_TOPOLOGY;
+                    }
+                    flags = REC_REFLUX;
+                    int icn[6];
+                    nbor_cells_of_cell(m, n, icn);
+                    nl = icn[2 * d];
+                    nr = icn[2 * d + 1];
+                }
+            }
+            out.rec_n.push_back(n);
+            out.rec_nl.push_back(nl);
+            out.rec_nr.push_back(nr);
+            out.rec_flags.push_back(flags);
+            int slot = -1;
+            if (flags & REC_REFLUX) {
+                slot = out.nslots;
+                out.nslots += 4;
+            }
+            out.rec_slot.push_back(slot);
+            out.nbo[(size_t)f * noct + p] = -1 - r;
+        }
+    }
+    // Mutual adjacency: the oct across face f of p must see p across the
+    // opposite face (true for every mesh the reference's TreeUpdater builds).
+    for (int p = 0; p < noct; ++p)
+        for (int f = 0; f < 6; ++f) {
+            int q = out.nbo[(size_t)f * noct + p];
+            if (q < 0) continue;
+            int back = out.nbo[(size_t)(f ^ 1) * noct + q];
+            if (back != p) {
+                err = "octs " + std::to_string(out.grid[p]) + " and " + std::to_string(out.grid[q]) +
+                      " are not mutual neighbours across face " + std::to_string(f);
+                return RH_ERR_TOPOLOGY;
+            }
+        }
+    // Coarse reflux targets, ordered per target by processing key.
+    struct Entry {
+        int target;
+        long long key;
+        int slot;
+    };
+    std::vector<Entry> ent;
+    for (int p = 0; p < noct; ++p)
+        for (int f = 0; f < 6; ++f) {
+            int q = out.nbo[(size_t)f * noct + p];
+            if (q >= 0) continue;
+            int r = -1 - q;
+            if (!(out.rec_flags[r] & REC_REFLUX)) continue;
+            int d = f >> 1, s = f & 1;
+            for (int ic0 = 0; ic0 < 8; ++ic0) {
+                if (!crosses(ic0, d, s)) continue;
+                if (!((out.leaf[p] >> ic0) & 1)) continue;
+                int j = ((ic0 >> (d + 1)) << d) | (ic0 & ((1 << d) - 1));
+                ent.push_back({out.rec_n[r], ((long long)p * 8 + ic0) * 8 + f, out.rec_slot[r] + j});
+            }
+        }
+    std::sort(ent.begin(), ent.end(), [](const Entry& a, const Entry& b) {
+        return a.target != b.target ? a.target < b.target : a.key < b.key;
+    });
+    for (size_t i = 0; i < ent.size(); ++i) {
+        if (i == 0 || ent[i].target != ent[i - 1].target) {
+            out.tgt_cell.push_back(ent[i].target);
+            out.tgt_off.push_back((int)i);
+        }
+        out.tgt_slot.push_back(ent[i].slot);
+    }
+    out.tgt_off.push_back((int)ent.size());
+    return RH_OK;
+}
+
+}  // namespace rh

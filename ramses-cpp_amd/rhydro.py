@@ -1,0 +1,326 @@
+"""ctypes binding of librhydro.so (include/rhydro.h, include/rhydro_driver.h).
+
+Python is plumbing here: it holds the AmrGrid-shaped numpy arrays of a run
+and calls the C-ABI.  All hydro arithmetic happens in the HIP kernels of the
+library.  Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librhydro.so")
+
+RH_OK, RH_ERR_ARG, RH_ERR_HIP, RH_ERR_UNSUPPORTED, RH_ERR_TOPOLOGY, RH_ERR_NO_DEVICE = range(6)
+RIEMANN = {"llf": 0, "hll": 1, "hllc": 2, "exact": 3, "acoustic": 4}
+RH_PREC_FP64, RH_PREC_MIXED = 0, 1
+RH_HOST_COHERENT, RH_DEVICE_RESIDENT = 0, 1
+RH_SYNC_TO_HOST, RH_SYNC_TO_DEVICE = 1, 2
+RH_UOLD, RH_UNEW = 1, 2
+TK_GODUNOV, TK_COURANT, TK_SET_UNEW, TK_SET_UOLD = range(4)
+
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+
+class RhParams(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double),
+        ("slope_type", C.c_int),
+        ("riemann", C.c_int),
+        ("interpol_type", C.c_int),
+        ("interpol_var", C.c_int),
+        ("nvar", C.c_int),
+        ("nener", C.c_int),
+        ("barotropic_eos", C.c_int),
+        ("boxlen", C.c_double),
+        ("precision", C.c_int),
+        ("device", C.c_int),
+    ]
+
+
+class RhMesh(C.Structure):
+    _fields_ = [
+        ("ncoarse", C.c_int), ("ngridmax", C.c_int), ("ncell", C.c_int), ("nvar", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("nlevelmax", C.c_int), ("ncpu", C.c_int), ("nboundary", C.c_int), ("myid", C.c_int),
+        ("father", _IP), ("son", _IP), ("nbor", _IP), ("next", _IP), ("headl", _IP),
+        ("uold", _DP), ("unew", _DP), ("f", _DP),
+        ("mesh_epoch", C.c_longlong),
+    ]
+
+
+def resolve_riemann(name: str | None) -> int:
+    """Namelist string -> solver id, as the reference's face loop resolves it
+    (HydroSolver.cpp:456-461): exact, case-sensitive matches, else LLF."""
+    return RIEMANN.get(name, 0) if name in ("hllc", "hll", "exact", "acoustic") else 0
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.rh_create.argtypes = [C.POINTER(RhParams), C.POINTER(C.c_void_p)]
+        L.rh_destroy.argtypes = [C.c_void_p]
+        L.rh_bind.argtypes = [C.c_void_p, C.POINTER(RhMesh), C.c_int]
+        L.rh_set_unew.argtypes = [C.c_void_p, C.c_int]
+        L.rh_set_uold.argtypes = [C.c_void_p, C.c_int]
+        L.rh_godunov_fine.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+        L.rh_courant.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, _DP]
+        L.rh_interpol_hydro.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
+        L.rh_sync.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.rh_last_error.argtypes = [C.c_void_p]
+        L.rh_last_error.restype = C.c_char_p
+        L.rh_run_uniform.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, _DP]
+        L.rh_uniform_ngrid.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        L.rh_uniform_ngrid.restype = C.c_longlong
+        L.rh_build_uniform.argtypes = [C.c_int] * 6 + [_IP] * 8 + [_DP]
+        L.rh_build_uniform.restype = C.c_longlong
+        L.rh_init_regions.argtypes = [C.POINTER(RhMesh), _DP, C.c_double, C.c_double, C.c_int, _IP] + [_DP] * 11
+        L.rh_init_lognormal.argtypes = [C.POINTER(RhMesh), C.c_int, C.c_ulonglong, C.c_double, C.c_double,
+                                        C.c_double]
+        L.rh_timers_enable.argtypes = [C.c_void_p, C.c_int]
+        L.rh_timers_read.argtypes = [C.c_void_p, C.c_int, _DP, C.POINTER(C.c_longlong)]
+        L.rh_timers_reset.argtypes = [C.c_void_p]
+        L.rh_level_leaf_cells.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
+        _lib = L
+    return _lib
+
+
+EXPORTED = [
+    "rh_create", "rh_destroy", "rh_bind", "rh_set_unew", "rh_godunov_fine", "rh_set_uold", "rh_courant",
+    "rh_interpol_hydro", "rh_sync", "rh_last_error", "rh_run_uniform", "rh_uniform_ngrid", "rh_build_uniform",
+    "rh_init_regions", "rh_init_lognormal", "rh_timers_enable", "rh_timers_read", "rh_timers_reset",
+    "rh_level_leaf_cells",
+]
+
+
+def _ip(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_IP)
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+@dataclass
+class Mesh:
+    """AmrGrid-shaped arrays (AmrGrid.hpp:15-33) for ncpu = 1."""
+    nx: int
+    ny: int
+    nz: int
+    ngridmax: int
+    nlevelmax: int
+    nvar: int = 5
+    epoch: int = 0
+    arrays: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        self.ncoarse = self.nx * self.ny * self.nz
+        self.ncell = self.ncoarse + 8 * self.ngridmax
+        a = self.arrays
+        a["father"] = np.zeros(self.ngridmax, np.int32)
+        a["son"] = np.zeros(self.ncell, np.int32)
+        a["nbor"] = np.zeros(6 * self.ngridmax, np.int32)
+        a["next"] = np.zeros(self.ngridmax, np.int32)
+        a["prev"] = np.zeros(self.ngridmax, np.int32)
+        a["headl"] = np.zeros(self.nlevelmax + 1, np.int32)
+        a["taill"] = np.zeros(self.nlevelmax + 1, np.int32)
+        a["numbl"] = np.zeros(self.nlevelmax + 1, np.int32)
+        a["xg"] = np.zeros(3 * self.ngridmax, np.float64)
+        a["uold"] = np.zeros(self.nvar * self.ncell, np.float64)
+        a["unew"] = np.zeros(self.nvar * self.ncell, np.float64)
+
+    def __getattr__(self, k):
+        arrays = self.__dict__.get("arrays")
+        if arrays is not None and k in arrays:
+            return arrays[k]
+        raise AttributeError(k)
+
+    def rh_mesh(self, f: np.ndarray | None = None) -> RhMesh:
+        a = self.arrays
+        m = RhMesh()
+        m.ncoarse, m.ngridmax, m.ncell, m.nvar = self.ncoarse, self.ngridmax, self.ncell, self.nvar
+        m.nx, m.ny, m.nz = self.nx, self.ny, self.nz
+        m.nlevelmax, m.ncpu, m.nboundary, m.myid = self.nlevelmax, 1, 0, 1
+        m.father, m.son, m.nbor, m.next, m.headl = (_ip(a[k]) for k in ("father", "son", "nbor", "next", "headl"))
+        m.uold, m.unew = _dp(a["uold"]), _dp(a["unew"])
+        m.f = _dp(f) if f is not None else C.cast(None, _DP)
+        m.mesh_epoch = self.epoch
+        return m
+
+    def level_octs(self, level: int) -> list[int]:
+        out, g = [], int(self.headl[level])
+        while g > 0:
+            out.append(g)
+            g = int(self.next[g - 1])
+        return out
+
+    def cells_of_level(self, level: int, leaf_only: bool = False) -> np.ndarray:
+        """0-based cell indices of the level's octs in processing order."""
+        octs = np.array(self.level_octs(level), dtype=np.int64)
+        ic = np.arange(8, dtype=np.int64)
+        cells = (self.ncoarse + ic[None, :] * self.ngridmax + octs[:, None] - 1).reshape(-1)
+        if leaf_only:
+            cells = cells[self.son[cells] == 0]
+        return cells
+
+    def state(self, which: str = "uold") -> np.ndarray:
+        return self.arrays[which].reshape(self.nvar, self.ncell)
+
+
+def uniform_mesh(levelmin: int, nx: int = 1, ny: int = 1, nz: int = 1, ngridmax: int | None = None,
+                 nlevelmax: int | None = None) -> Mesh:
+    """The uniform mesh the reference builds for levelmin == levelmax."""
+    L = lib()
+    need = int(L.rh_uniform_ngrid(nx, ny, nz, levelmin))
+    ngridmax = max(need, 1) if ngridmax is None else ngridmax
+    nlevelmax = levelmin if nlevelmax is None else nlevelmax
+    m = Mesh(nx, ny, nz, ngridmax, nlevelmax)
+    a = m.arrays
+    ng = L.rh_build_uniform(nx, ny, nz, levelmin, nlevelmax, ngridmax, _ip(a["father"]), _ip(a["son"]),
+                            _ip(a["nbor"]), _ip(a["next"]), _ip(a["prev"]), _ip(a["headl"]), _ip(a["taill"]),
+                            _ip(a["numbl"]), _dp(a["xg"]))
+    if ng < 0:
+        raise RuntimeError(f"rh_build_uniform failed ({ng})")
+    return m
+
+
+def init_regions(m: Mesh, regions: list[dict], gamma: float, boxlen: float = 1.0) -> None:
+    """regions: dicts with type ('square'/'cube'/'sphere'), x/y/z_center,
+    length_x/y/z, d, u, v, w, p (Initializer.cpp defaults for missing keys)."""
+    n = len(regions)
+
+    def col(key, default):
+        return np.array([float(r.get(key, default)) for r in regions], np.float64)
+
+    types = np.array([1 if r.get("type", "cube") == "sphere" else 0 for r in regions], np.int32)
+    cols = [col("x_center", 0.5), col("y_center", 0.5), col("z_center", 0.5), col("length_x", 1.0),
+            col("length_y", 1.0), col("length_z", 1.0), col("d", 1.0), col("u", 0.0), col("v", 0.0),
+            col("w", 0.0), col("p", 1.0)]
+    mm = m.rh_mesh()
+    st = lib().rh_init_regions(C.byref(mm), _dp(m.xg), boxlen, gamma, n, _ip(types), *[_dp(c) for c in cols])
+    if st != RH_OK:
+        raise RuntimeError(f"rh_init_regions failed ({st})")
+
+
+def init_lognormal(m: Mesh, level: int, seed: int, gamma: float, sigma: float = 0.5, vel_amp: float = 0.5) -> None:
+    mm = m.rh_mesh()
+    st = lib().rh_init_lognormal(C.byref(mm), level, seed, gamma, sigma, vel_amp)
+    if st != RH_OK:
+        raise RuntimeError(f"rh_init_lognormal failed ({st})")
+
+
+def sedov_regions(levelmin: int, nx: int = 1, gamma: float = 1.4, energy: float = 1.0, p0: float = 1e-5,
+                  boxlen: float = 1.0) -> list[dict]:
+    """Sedov set-up of the benchmark: uniform rho=1, p=p0 square, and the
+    energy deposited as pressure in the cells with r < 1.5 h (8 cells)."""
+    h = boxlen / (nx * 2 ** (levelmin - 1))
+    p_in = (gamma - 1.0) * energy / (8 * h ** 3)
+    return [
+        dict(type="square", x_center=0.5, y_center=0.5, z_center=0.5, length_x=10.0, length_y=10.0,
+             length_z=10.0, d=1.0, p=p0),
+        dict(type="sphere", x_center=0.5, y_center=0.5, z_center=0.5, length_x=1.5 * h, d=1.0, p=p_in),
+    ]
+
+
+class RhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rhydro error {code}: {msg}")
+        self.code = code
+
+
+class Solver:
+    """One rh_ctx bound to one Mesh (the HydroSolver of the reference)."""
+
+    def __init__(self, gamma=1.4, slope_type=1, riemann="hllc", interpol_type=1, interpol_var=0, boxlen=1.0,
+                 precision=RH_PREC_FP64, device=0):
+        self.params = RhParams(gamma, slope_type, riemann if isinstance(riemann, int) else resolve_riemann(riemann),
+                               interpol_type, interpol_var, 5, 0, 0, boxlen, precision, device)
+        self._ctx = C.c_void_p()
+        st = lib().rh_create(C.byref(self.params), C.byref(self._ctx))
+        if st != RH_OK:
+            raise RhError(st, lib().rh_last_error(None).decode())
+        self.mesh = None
+        self._rhm = None
+        self._f = None
+
+    def _chk(self, st):
+        if st != RH_OK:
+            raise RhError(st, lib().rh_last_error(self._ctx).decode())
+
+    def bind(self, mesh: Mesh, mode=RH_DEVICE_RESIDENT, f: np.ndarray | None = None):
+        self.mesh, self._f = mesh, f
+        self._rhm = mesh.rh_mesh(f)
+        self._chk(lib().rh_bind(self._ctx, C.byref(self._rhm), mode))
+
+    def set_unew(self, ilevel):
+        self._chk(lib().rh_set_unew(self._ctx, ilevel))
+
+    def set_uold(self, ilevel):
+        self._chk(lib().rh_set_uold(self._ctx, ilevel))
+
+    def godunov_fine(self, ilevel, dt, dx):
+        self._chk(lib().rh_godunov_fine(self._ctx, ilevel, dt, dx))
+
+    def courant(self, ilevel, dx, gamma, courant_factor) -> float:
+        out = C.c_double()
+        self._chk(lib().rh_courant(self._ctx, ilevel, dx, gamma, courant_factor, C.byref(out)))
+        return out.value
+
+    def run_uniform(self, ilevel, dx, courant_factor, nsteps) -> np.ndarray:
+        dts = np.zeros(max(nsteps, 1), np.float64)
+        self._chk(lib().rh_run_uniform(self._ctx, ilevel, dx, courant_factor, nsteps, _dp(dts)))
+        return dts[:nsteps]
+
+    def interpol(self, u1: np.ndarray, u2: np.ndarray | None = None) -> np.ndarray:
+        u1 = np.ascontiguousarray(u1, np.float64)
+        n = u1.shape[0]
+        assert u1.shape == (n, 7, 64)
+        u2 = np.zeros((n, 8, 64), np.float64) if u2 is None else np.ascontiguousarray(u2, np.float64)
+        self._chk(lib().rh_interpol_hydro(self._ctx, n, _dp(u1), _dp(u2)))
+        return u2
+
+    def sync_to_host(self, what=RH_UOLD | RH_UNEW):
+        self._chk(lib().rh_sync(self._ctx, RH_SYNC_TO_HOST, what))
+
+    def sync_to_device(self, what=RH_UOLD | RH_UNEW):
+        self._chk(lib().rh_sync(self._ctx, RH_SYNC_TO_DEVICE, what))
+
+    def timers(self, enable=True):
+        self._chk(lib().rh_timers_enable(self._ctx, 1 if enable else 0))
+
+    def timer(self, kernel) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_longlong()
+        self._chk(lib().rh_timers_read(self._ctx, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def timers_reset(self):
+        self._chk(lib().rh_timers_reset(self._ctx))
+
+    def leaf_cells(self, ilevel) -> int:
+        n = C.c_longlong()
+        self._chk(lib().rh_level_leaf_cells(self._ctx, ilevel, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if self._ctx:
+            lib().rh_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
