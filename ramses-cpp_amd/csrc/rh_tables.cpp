@@ -87,6 +87,17 @@ void nbor_cells_of_cell(const rh_mesh& m, int icell, int icn[6]) {
 
 int build_topo_index(const rh_mesh& m, TopoIndex& ti, std::string& err) {
     MeshView v{m};
+    // Every index the device kernels will follow must stay inside its array.
+    for (int i = 0; i < m.ncell; ++i)
+        if (m.son[i] < 0 || m.son[i] > m.ngridmax) {
+            err = "son[" + std::to_string(i + 1) + "] = " + std::to_string(m.son[i]) + " outside 0..ngridmax";
+            return RH_ERR_TOPOLOGY;
+        }
+    for (int g = 0; g < m.ngridmax; ++g)
+        if (m.father[g] < 0 || m.father[g] > m.ncell || m.next[g] < 0 || m.next[g] > m.ngridmax) {
+            err = "father/next of grid " + std::to_string(g + 1) + " out of range";
+            return RH_ERR_TOPOLOGY;
+        }
     ti.glev.assign((size_t)m.ngridmax + 1, 0);
     ti.gpos.assign((size_t)m.ngridmax + 1, -1);
     ti.depth.assign((size_t)m.ngridmax + 1, 0);
