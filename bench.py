@@ -50,6 +50,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true", help="skip the mixed-precision form")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -69,39 +70,98 @@ def main() -> int:
     print(f"[bench] mesh level {level}: {mesh.ngridmax} grids, {mesh.ncell} cells, setup "
           f"{time.time() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
-    s = rhydro.Solver(gamma=gamma, slope_type=args.slope, riemann=args.riemann, precision=rhydro.RH_PREC_FP64)
-    s.bind(mesh, rhydro.RH_DEVICE_RESIDENT)
-    nleaf = s.leaf_cells(level)
     dx = 1.0 / (2 ** level)  # Simulation.cpp:548, boxlen/(nx*2^ilevel)
     cf = 0.8
+    u0 = mesh.uold.copy()
 
-    s.run_uniform(level, dx, cf, args.warmup)  # ends with a stream synchronisation
-    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
-    s.timers(True)
-    s.timers_reset()
-    t0 = time.perf_counter()
-    dts = s.run_uniform(level, dx, cf, args.steps)  # synchronises before returning
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    g_ms, g_n = s.timer(rhydro.TK_GODUNOV)
-    c_ms, _ = s.timer(rhydro.TK_COURANT)
-    u_ms, _ = s.timer(rhydro.TK_SET_UNEW)
-    o_ms, _ = s.timer(rhydro.TK_SET_UOLD)
-    s.timers(False)
-    print(f"[bench] timed {args.steps} steps: {wall:.3f}s; dt[0..2]={dts[:3].tolist()}", file=sys.stderr,
-          flush=True)
+    def measure(precision):
+        """Warmup, then exactly `steps` timed steps of one precision form."""
+        mesh.uold[:] = u0
+        mesh.unew[:] = u0
+        s = rhydro.Solver(gamma=gamma, slope_type=args.slope, riemann=args.riemann, precision=precision)
+        s.bind(mesh, rhydro.RH_DEVICE_RESIDENT)
+        nleaf = s.leaf_cells(level)
+        warm_dts = s.run_uniform(level, dx, cf, max(args.warmup, 1))  # ends with a stream synchronisation
+        # Pick the faster Godunov kernel variant on this level (results are
+        # identical for every variant; unew is restored by the tuner).
+        best, tune_ms = s.autotune(level, float(warm_dts[-1]), dx, reps=2)
+        s.timers(True)
+        s.timers_reset()
+        t0 = time.perf_counter()
+        dts = s.run_uniform(level, dx, cf, args.steps)  # synchronises before returning
+        wall = time.perf_counter() - t0
+        k = {name: s.timer(tk) for name, tk in (("godunov", rhydro.TK_GODUNOV), ("courant", rhydro.TK_COURANT),
+                                                ("set_unew", rhydro.TK_SET_UNEW), ("set_uold", rhydro.TK_SET_UOLD))}
+        s.timers(False)
+        s.sync_to_host()
+        s.close()
+        god_avg_s = k["godunov"][0] / max(k["godunov"][1], 1) / 1e3
+        achieved = BYTES_PER_CELL_UPDATE * nleaf / god_avg_s / 1e9
+        tag = "fp64" if precision == rhydro.RH_PREC_FP64 else "mixed"
+        print(f"[bench] {tag}: {args.steps} steps in {wall:.4f}s, godunov {god_avg_s * 1e3:.3f} ms", file=sys.stderr,
+              flush=True)
+        return dict(nleaf=nleaf, wall=wall, dts=dts, achieved=achieved, final=mesh.uold.copy(),
+                    kernel_ms={n: v[0] / max(v[1], 1) for n, v in k.items()},
+                    autotune={"chosen_variant": best, "godunov_ms_by_variant": tune_ms.tolist()})
 
-    value = nleaf * args.steps / wall
-    god_avg_s = (g_ms / max(g_n, 1)) / 1e3
-    achieved = BYTES_PER_CELL_UPDATE * nleaf / god_avg_s / 1e9
+    fp64 = measure(rhydro.RH_PREC_FP64)
+    mixed = measure(rhydro.RH_PREC_MIXED) if not args.fp64_only else None
+    nleaf = fp64["nleaf"]
+
+    def one_step(precision):
+        mesh.uold[:] = u0
+        mesh.unew[:] = u0
+        s = rhydro.Solver(gamma=gamma, slope_type=args.slope, riemann=args.riemann, precision=precision)
+        s.bind(mesh, rhydro.RH_DEVICE_RESIDENT)
+        s.run_uniform(level, dx, cf, 1)
+        s.sync_to_host()
+        s.close()
+        return mesh.uold.copy()
+
+    def errors(a, b):
+        """L1 (mean) and Linf relative errors of rho, p, |v| of state a against b."""
+        cells = mesh.cells_of_level(level)
+        A, B = a.reshape(5, -1)[:, cells], b.reshape(5, -1)[:, cells]
+        out = {}
+        va, vb = A[1:4] / A[0], B[1:4] / B[0]
+        pa = (gamma - 1) * (A[4] - 0.5 * A[0] * (va ** 2).sum(0))
+        pb = (gamma - 1) * (B[4] - 0.5 * B[0] * (vb ** 2).sum(0))
+        sa, sb = np.sqrt((va ** 2).sum(0)), np.sqrt((vb ** 2).sum(0))
+        for name, x, y, scale in (("rho", A[0], B[0], np.abs(B[0])), ("p", pa, pb, np.abs(pb)),
+                                  ("v", sa, sb, np.full_like(sb, sb.max()))):
+            r = np.abs(x - y) / scale
+            out[name] = {"L1": float(r.mean()), "Linf": float(r.max())}
+        return out
+
+    mixed_info = None
+    if mixed is not None:
+        err1 = errors(one_step(rhydro.RH_PREC_MIXED), one_step(rhydro.RH_PREC_FP64))
+        errn = errors(mixed["final"], fp64["final"])
+        mixed_info = {
+            "value": nleaf * args.steps / mixed["wall"],
+            "ms_per_step": mixed["wall"] / args.steps * 1e3,
+            "kernel_ms": mixed["kernel_ms"],
+            "autotune": mixed["autotune"],
+            "roofline_frac": mixed["achieved"] / HBM_PEAK_GBS,
+            "speedup_vs_fp64": fp64["wall"] / mixed["wall"],
+            "error_vs_fp64_one_step": err1,
+            "error_vs_fp64_after_steps": errn,
+            "error_note": "relative errors of the mixed form against the FP64 form (bit-identical to the "
+                          "reference restatement) on leaf cells; one step from identical input, and after "
+                          f"{args.warmup}+{args.steps} steps with each form's own time steps",
+        }
+
+    value = nleaf * args.steps / fp64["wall"]
+    achieved = fp64["achieved"]
+    g_ms = fp64["kernel_ms"]["godunov"]
 
     cpu = None
     if not args.no_cpu and args.cpu_seconds > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
 
-        s.sync_to_host()
-        s.close()
+        mesh.uold[:] = fp64["final"]
+        mesh.unew[:] = fp64["final"]
         om = oracle.OracleMesh(mesh)
         p = oracle.params(gamma, args.slope, rhydro.resolve_riemann(args.riemann), 1, 0, 1.0)
         sample_octs = 1 << 14  # 131,072 cells per call (a 64x64x32 Morton block)
@@ -109,7 +169,7 @@ def main() -> int:
         tc0 = time.perf_counter()
         calls = 0
         while True:
-            om.godunov_fine_sample(p, level, float(dts[-1]), dx, sample_octs)
+            om.godunov_fine_sample(p, level, float(fp64["dts"][-1]), dx, sample_octs)
             cells += sample_octs * 8
             calls += 1
             if time.perf_counter() - tc0 >= args.cpu_seconds:
@@ -124,8 +184,6 @@ def main() -> int:
                       f"no FMA) on the first {sample_octs} octs ({sample_octs * 8} leaf cells) of the same "
                       f"level-{level} state, {tcpu:.1f}s on {cpu_model()}",
         }
-    else:
-        s.close()
 
     out = {
         "metric": METRIC,
@@ -134,7 +192,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": fp64["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -149,8 +207,9 @@ def main() -> int:
             "precision": "fp64",
             "riemann": args.riemann,
             "slope_type": args.slope,
-            "kernel_ms": {"godunov": g_ms / max(g_n, 1), "courant": c_ms / max(args.steps, 1),
-                          "set_unew": u_ms / max(args.steps, 1), "set_uold": o_ms / max(args.steps, 1)},
+            "kernel_ms": fp64["kernel_ms"],
+            "autotune": fp64["autotune"],
+            "mixed": mixed_info,
         },
         "roofline": {
             "bound": "hbm",

@@ -75,6 +75,8 @@ def lib() -> C.CDLL:
         L.or_prim_to_cons.argtypes = [_DP, _DP, C.c_double]
         L.or_compute_flux.argtypes = [_DP, _DP, C.c_double]
         L.or_build_uniform.argtypes = [C.c_int] * 6 + [_IP] * 8
+        L.or_refine.argtypes = [C.POINTER(OrMesh), C.POINTER(OrParams), C.c_int, _IP, C.c_int, _IP, _IP, _IP,
+                                _DP, C.POINTER(C.c_int), _IP, _IP, _IP, _IP, _IP]
         _lib = L
     return _lib
 
@@ -207,3 +209,23 @@ def build_uniform(nx, ny, nz, levelmin, nlevelmax, ngridmax):
     if used < 0:
         raise RuntimeError("ngridmax too small")
     return a, used
+
+
+def refine(m, p: OrParams, ilevel: int, cells) -> int:
+    """Refines the given 1-based cells of level list `ilevel` the way
+    TreeUpdater::make_grid_fine does (see or_refine); `m` is a mesh object
+    with the AmrGrid arrays, a `headf` attribute (first free grid) and an
+    `epoch` that is bumped."""
+    cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int32))
+    om = OracleMesh(m)
+    headf = C.c_int(int(m.headf))
+    n = lib().or_refine(C.byref(om.s), C.byref(p), ilevel, _ip(cells), int(cells.size), _ip(m.prev), _ip(m.taill),
+                        _ip(m.numbl), _dp(m.xg), C.byref(headf), _ip(m.son), _ip(m.father), _ip(m.nbor),
+                        _ip(m.next), _ip(m.headl))
+    if n == -2:
+        raise ValueError(f"level list {ilevel + 1} is beyond nlevelmax={m.nlevelmax}")
+    if n < 0:
+        raise RuntimeError("free list exhausted")
+    m.headf = headf.value
+    m.epoch += 1
+    return n

@@ -1159,3 +1159,96 @@ int or_build_uniform(int nx, int ny, int nz, int levelmin, int nlevelmax, int ng
 }
 
 }  // extern "C"
+
+// ---- refinement of flagged cells ------------------------------------------
+// TreeUpdater::make_grid_fine (TreeUpdater.cpp:27-150) for an explicit set of
+// cells of level list `ilevel` (1-based cell ids, in the order the reference
+// would visit them: list order, then cell order) instead of flag1/flag2.
+// New grids come from the free list starting at *headf (AmrGrid.cpp:259-268),
+// children get the prolongation of interpol_hydro (with u1 built from the
+// parent cell and its six neighbours, TreeUpdater.cpp:103-138) in both uold
+// and unew, and each new oct is appended to level list ilevel+1.  Assumes the
+// free list has room (no resize).  xg is updated as TreeUpdater.cpp:78-101.
+// Returns the number of octs created, -1 when the free list ran out, -2 when
+// level list ilevel+1 is beyond nlevelmax.
+extern "C" int or_refine(or_mesh* mm, const or_params* p, int ilevel, const int* cells, int ncells, int* prev,
+                         int* taill, int* numbl, double* xg, int* headf, int* son_w, int* father_w, int* nbor_w,
+                         int* next_w, int* headl_w) {
+    or_mesh& m = *mm;
+    Ctx c{&m, p};
+    if (ilevel < 1 || ilevel + 1 > m.nlevelmax) return -2;  // list ilevel+1 must exist
+    int created = 0;
+    auto get_free = [&]() {
+        int ig = *headf;
+        if (ig == 0) return 0;
+        *headf = next_w[ig - 1];
+        if (*headf != 0) prev[*headf - 1] = 0;
+        next_w[ig - 1] = 0;
+        prev[ig - 1] = 0;
+        return ig;
+    };
+    auto add_to_list = [&](int ig, int il) {
+        if (headl_w[il] == 0) {
+            headl_w[il] = ig;
+            taill[il] = ig;
+            prev[ig - 1] = 0;
+            next_w[ig - 1] = 0;
+        } else {
+            int tail = taill[il];
+            next_w[tail - 1] = ig;
+            prev[ig - 1] = tail;
+            next_w[ig - 1] = 0;
+            taill[il] = ig;
+        }
+        numbl[il]++;
+    };
+    for (int k = 0; k < ncells; ++k) {
+        int ic_coarse = cells[k];
+        int idc = ic_coarse - 1;
+        if (son_w[idc] != 0) continue;
+        int new_ig = get_free();
+        if (new_ig == 0) return -1;
+        created++;
+        son_w[idc] = new_ig;
+        father_w[new_ig - 1] = idc + 1;
+        int icn_nb[6] = {0, 0, 0, 0, 0, 0};
+        if (ilevel == 1) {
+            c.nbor_cells_coarse(ic_coarse, icn_nb);
+            for (int f = 0; f < 6; ++f) nbor_w[f * m.ngridmax + new_ig - 1] = icn_nb[f];
+            int idx = ic_coarse - 1;
+            int x[3] = {idx % m.nx, (idx % (m.nx * m.ny)) / m.nx, idx / (m.nx * m.ny)};
+            for (int d = 0; d < 3; ++d) xg[d * m.ngridmax + new_ig - 1] = (double)x[d] + 0.5;
+        } else {
+            int ig = ((idc - m.ncoarse) % m.ngridmax) + 1;
+            int ic = ((idc - m.ncoarse) / m.ngridmax) + 1;
+            int ign[7];
+            c.nbor_grids(ig, ign);
+            c.nbor_cells(ign, ic, icn_nb, ig);
+            for (int f = 0; f < 6; ++f) nbor_w[f * m.ngridmax + new_ig - 1] = icn_nb[f];
+            double dx_level = 1.0 / (double)(1 << (ilevel - 1));
+            int ixyz[3] = {(ic - 1) & 1, ((ic - 1) & 2) >> 1, ((ic - 1) & 4) >> 2};
+            for (int d = 0; d < 3; ++d)
+                xg[d * m.ngridmax + new_ig - 1] = xg[d * m.ngridmax + ig - 1] + (double)(ixyz[d] - 0.5) * dx_level;
+        }
+        double u1[7][64] = {{0}}, u2[8][64] = {{0}};
+        for (int iv = 1; iv <= NV; ++iv) u1[0][iv - 1] = c.uold(ic_coarse, iv);
+        for (int i = 1; i <= 6; ++i) {
+            int id_n = icn_nb[i - 1];
+            if (id_n > 0) {
+                for (int iv = 1; iv <= NV; ++iv) u1[i][iv - 1] = c.uold(id_n, iv);
+            } else {
+                for (int iv = 1; iv <= NV; ++iv) u1[i][iv - 1] = u1[0][iv - 1];
+            }
+        }
+        or_interpol_hydro(p, NV, &u1[0][0], &u2[0][0]);
+        for (int isc = 1; isc <= 8; ++isc) {
+            int id_child = m.ncoarse + (isc - 1) * m.ngridmax + new_ig - 1;
+            for (int iv = 1; iv <= NV; ++iv) {
+                m.uold[(size_t)(iv - 1) * m.ncell + id_child] = u2[isc - 1][iv - 1];
+                m.unew[(size_t)(iv - 1) * m.ncell + id_child] = u2[isc - 1][iv - 1];
+            }
+        }
+        add_to_list(new_ig, ilevel + 1);
+    }
+    return created;
+}

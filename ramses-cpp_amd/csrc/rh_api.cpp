@@ -55,6 +55,7 @@ struct rh_ctx {
     double* d_u1 = nullptr;
     double* d_u2 = nullptr;
     int interpol_cap = 0;
+    int variant = 1;  // Godunov kernel variant (rh_set_kernel_variant)
     bool timers = false;
     double t_ms[TK_N] = {0, 0, 0, 0};
     long long t_n[TK_N] = {0, 0, 0, 0};
@@ -284,8 +285,8 @@ int do_courant(rh_ctx* c, const rh::LevelDev* lv, double dx, double gamma, doubl
 
 int do_godunov(rh_ctx* c, const rh::LevelDev* lv, const double* d_dt, double dx) {
     Timed tm(c, TK_GODUNOV);
-    hipError_t e = rh::launch_godunov(mesh_dev(c), *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision, d_dt, dx,
-                                      c->stream);
+    hipError_t e = rh::launch_godunov(mesh_dev(c), *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision,
+                                      c->variant, d_dt, dx, c->stream);
     if (e == hipErrorNotSupported) return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
     if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
     tm.done();
@@ -513,6 +514,71 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
                                  c->stream));
     RH_HIP(c, hipStreamSynchronize(c->stream));
     return post_call(c, RH_UOLD | RH_UNEW);
+}
+
+int rh_set_kernel_variant(rh_ctx* c, int variant) {
+    if (!c) return RH_ERR_ARG;
+    if (variant != 0 && variant != 1) return fail(c, RH_ERR_ARG, "unknown kernel variant");
+    c->variant = variant;
+    return RH_OK;
+}
+
+int rh_autotune(rh_ctx* c, int ilevel, double dt, double dx, int reps, int* best_variant, double* ms_per_variant) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (reps < 1) return fail(c, RH_ERR_ARG, "reps < 1");
+    if ((st = pre_call(c))) return st;
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    if ((st = ensure_dt(c, 1))) return st;
+    RH_HIP(c, hipMemcpyAsync(c->d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // Every variant computes the same unew; a saved copy is restored after
+    // each trial so the bound state is left as it was.
+    const size_t bytes = state_bytes(c);
+    double* saved = nullptr;
+    RH_HIP(c, hipMalloc(&saved, bytes));
+    RH_HIP(c, hipMemcpyAsync(saved, c->d_unew, bytes, hipMemcpyDeviceToDevice, c->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int keep = c->variant;
+    int best = keep;
+    float best_ms = 0.f;
+    int rc = RH_OK;
+    for (int v = 0; v < 2 && rc == RH_OK; ++v) {
+        c->variant = v;
+        float total = 0.f;
+        for (int r = 0; r <= reps && rc == RH_OK; ++r) {  // first launch untimed
+            (void)hipEventRecord(e0, c->stream);
+            rc = do_godunov(c, lv, c->d_dt, dx);
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (r > 0) total += ms;
+            if (hipMemcpyAsync(c->d_unew, saved, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                rc = fail(c, RH_ERR_HIP, "restore unew");
+        }
+        const float avg = total / reps;
+        if (ms_per_variant) ms_per_variant[v] = avg;
+        if (v == 0 || avg < best_ms) {
+            best_ms = avg;
+            best = v;
+        }
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(saved);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->variant = rc == RH_OK ? best : keep;
+    if (best_variant) *best_variant = c->variant;
+    return rc;
+}
+
+int rh_get_kernel_variant(rh_ctx* c, int* variant) {
+    if (!c || !variant) return RH_ERR_ARG;
+    *variant = c->variant;
+    return RH_OK;
 }
 
 int rh_timers_enable(rh_ctx* c, int enable) {

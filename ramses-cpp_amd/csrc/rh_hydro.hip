@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "rh_kernels.h"
 #include "rh_physics.h"
@@ -147,187 +148,42 @@ struct Nbr {
     int rec;   // record index when pos < 0
 };
 
-template <int RIEMANN>
-__global__ __launch_bounds__(BLOCK) void k_godunov_fp64(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
-                                                         double dx) {
-    const int t = threadIdx.x;
-    const int p = blockIdx.x * OCTS_PER_BLOCK + (t % OCTS_PER_BLOCK);
-    const int ic0 = t / OCTS_PER_BLOCK;
-    if (p >= lv.noct) return;
-    if (!((lv.leaf[p] >> ic0) & 1)) return;
-
-    const Gas<double> g{gp.gamma, gp.gm1, gp.smalle, gp.smallp};
-    const int slope_type = gp.slope_type, interpol_type = gp.interpol_type;
-    const double dt = *d_dt;
-    const double dtdx = dt / dx;
-    const int noct = lv.noct;
-    const int cC = cell_id(m, lv.grid[p], ic0);
-
-    double uC[5], qC[5];
-    load_u(m.uold, m.ncell, cC, uC);
-    ctoprim(uC, qC, g);
-
-    double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double rin[6][5];
-    long long kin[6];
-
-    auto same = [&](int pos, int ic) {
-        Nbr n;
-        n.cell = cell_id(m, lv.grid[pos], ic);
-        n.pos = pos;
-        n.ic0 = ic;
-        n.rec = -1;
-        return n;
-    };
-    auto from_rec = [&](int code) {
-        Nbr n;
-        n.rec = -1 - code;
-        n.cell = lv.rec[n.rec].x;
-        n.pos = -1;
-        n.ic0 = 0;
-        return n;
-    };
-    // Primitive state of a cell as read by the slope / ghost stencils: a
-    // boundary index falls back to `fallback` (the reference copies the
-    // centre cell's state there; bound_type is never filled).
-    auto prim = [&](int icell, const double fallback[5], double q[5]) {
-        if (icell > 0) {
-            double u[5];
-            load_u(m.uold, m.ncell, icell, u);
-            ctoprim(u, q, g);
-        } else {
-            copy5(fallback, q);
-        }
-    };
-
-#pragma unroll 1
-    for (int d = 0; d < 3; ++d) {
-        const int b = 1 << d;
-        const int par = (ic0 >> d) & 1;
-        const int A = lv.nbo[(size_t)(2 * d) * noct + p];
-        const int B = lv.nbo[(size_t)(2 * d + 1) * noct + p];
-        const Nbr L1 = par ? same(p, ic0 ^ b) : (A >= 0 ? same(A, ic0 ^ b) : from_rec(A));
-        const Nbr R1 = par ? (B >= 0 ? same(B, ic0 ^ b) : from_rec(B)) : same(p, ic0 ^ b);
-
-        double qL1[5], qR1[5];
-        prim(L1.cell, qC, qL1);
-        prim(R1.cell, qC, qR1);
-
-        // Own face states (trace pass) and this cell's ghost states as its
-        // neighbours see them (coarse-ghost branch with n = this cell).
-        double OCm[5], OCp[5], GCm[5], GCp[5];
-        own_trace(qL1, qC, qR1, d, slope_type, dt, dx, OCm, OCp, g);
-        ghost_trace(qL1, qC, qR1, d, interpol_type, dt, dx, GCm, GCp, g);
-
-        // ---- minus face (side 0): ql = ghost qm of L1, qr = own qp
-        {
-            double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            long long K = -1;
-            if (L1.pos >= 0) {
-                if ((lv.leaf[L1.pos] >> L1.ic0) & 1) {
-                    double qL2[5];
-                    const int L2cell = A >= 0 ? cell_id(m, lv.grid[A], ic0) : lv.rec[-1 - A].x;
-                    prim(L2cell, qL1, qL2);
-                    double Gm[5], Gp[5];
-                    ghost_trace(qL2, qL1, qC, d, interpol_type, dt, dx, Gm, Gp, g);
-                    riemann<RIEMANN>(Gm, OCp, F, g);
-                    rotate(F, d);
-                    // L1's own plus-face flux into this cell (reflux).
-                    double Om[5], Op[5], Fin[5];
-                    own_trace(qL2, qL1, qC, d, slope_type, dt, dx, Om, Op, g);
-                    riemann<RIEMANN>(Om, GCp, Fin, g);
-                    rotate(Fin, d);
+template <typename TC> RH_DEV void narrow(const double a[5], TC b[5]) {
 #pragma unroll
-                    for (int iv = 0; iv < 5; ++iv) rin[2 * d][iv] = dtdx * -1.0 * Fin[iv] * 0.125;
-                    K = ((long long)L1.pos * 8 + L1.ic0) * 8 + (2 * d + 1);
-                }
-            } else {
-                const int4 rc = lv.rec[L1.rec];
-                if (rc.w & REC_REFINED_DEV) {
-                } else if (rc.w & REC_BOUNDARY_DEV) {
-                    double Bm[5], Bp[5];
-                    boundary_trace(uC, true, d, dt, dx, Bm, Bp, g);
-                    riemann<RIEMANN>(Bm, OCp, F, g);
-                    rotate(F, d);
-                } else {
-                    double qn[5], qnl[5], qnr[5];
-                    prim(rc.x, qC, qn);
-                    prim(rc.y, qn, qnl);
-                    prim(rc.z, qn, qnr);
-                    double Gm[5], Gp[5];
-                    ghost_trace(qnl, qn, qnr, d, interpol_type, dt, dx, Gm, Gp, g);
-                    riemann<RIEMANN>(Gm, OCp, F, g);
-                    rotate(F, d);
-                    const int slot = lv.rec_slot[L1.rec] + face_slot(ic0, d);
-#pragma unroll
-                    for (int iv = 0; iv < 5; ++iv) lv.slot_r[(size_t)slot * 5 + iv] = dtdx * 1.0 * F[iv] * 0.125;
-                }
-            }
-#pragma unroll
-            for (int iv = 0; iv < 5; ++iv) fs[iv] = fs[iv] + 1.0 * F[iv];
-            kin[2 * d] = K;
-        }
-        // ---- plus face (side 1): ql = own qm, qr = ghost qp of R1
-        {
-            double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            long long K = -1;
-            if (R1.pos >= 0) {
-                if ((lv.leaf[R1.pos] >> R1.ic0) & 1) {
-                    double qR2[5];
-                    const int R2cell = B >= 0 ? cell_id(m, lv.grid[B], ic0) : lv.rec[-1 - B].x;
-                    prim(R2cell, qR1, qR2);
-                    double Gm[5], Gp[5];
-                    ghost_trace(qC, qR1, qR2, d, interpol_type, dt, dx, Gm, Gp, g);
-                    riemann<RIEMANN>(OCm, Gp, F, g);
-                    rotate(F, d);
-                    // R1's own minus-face flux into this cell (reflux).
-                    double Om[5], Op[5], Fin[5];
-                    own_trace(qC, qR1, qR2, d, slope_type, dt, dx, Om, Op, g);
-                    riemann<RIEMANN>(GCm, Op, Fin, g);
-                    rotate(Fin, d);
-#pragma unroll
-                    for (int iv = 0; iv < 5; ++iv) rin[2 * d + 1][iv] = dtdx * 1.0 * Fin[iv] * 0.125;
-                    K = ((long long)R1.pos * 8 + R1.ic0) * 8 + (2 * d);
-                }
-            } else {
-                const int4 rc = lv.rec[R1.rec];
-                if (rc.w & REC_REFINED_DEV) {
-                } else if (rc.w & REC_BOUNDARY_DEV) {
-                    double Bm[5], Bp[5];
-                    boundary_trace(uC, true, d, dt, dx, Bm, Bp, g);
-                    riemann<RIEMANN>(OCm, Bp, F, g);
-                    rotate(F, d);
-                } else {
-                    double qn[5], qnl[5], qnr[5];
-                    prim(rc.x, qC, qn);
-                    prim(rc.y, qn, qnl);
-                    prim(rc.z, qn, qnr);
-                    double Gm[5], Gp[5];
-                    ghost_trace(qnl, qn, qnr, d, interpol_type, dt, dx, Gm, Gp, g);
-                    riemann<RIEMANN>(OCm, Gp, F, g);
-                    rotate(F, d);
-                    const int slot = lv.rec_slot[R1.rec] + face_slot(ic0, d);
-#pragma unroll
-                    for (int iv = 0; iv < 5; ++iv) lv.slot_r[(size_t)slot * 5 + iv] = dtdx * -1.0 * F[iv] * 0.125;
-                }
-            }
-#pragma unroll
-            for (int iv = 0; iv < 5; ++iv) fs[iv] = fs[iv] + -1.0 * F[iv];
-            kin[2 * d + 1] = K;
-        }
-    }
+    for (int i = 0; i < 5; ++i) b[i] = (TC)a[i];
+}
 
+// One face's Riemann problem: the FP64 form calls the bit-faithful solver on
+// double states; the mixed form solves the wave structure in FP32 and
+// assembles the flux in FP64.  ql/qr are rotated; F comes back rotated.
+template <int RIEMANN, bool MIXED, typename TC>
+RH_DEV void solve_face(const TC ql[5], const TC qr[5], double F[5], const Gas<TC>& gc, const Gas<double>& gd) {
+    if constexpr (MIXED) riemann_mixed<RIEMANN>(ql, qr, F, gc, gd);
+    else riemann<RIEMANN>(ql, qr, F, gd);
+}
+
+// Octs per workgroup of the Godunov kernel (8 threads per oct).
+constexpr int G_OCTS = 16;
+constexpr int G_BLOCK = G_OCTS * 8;
+
+// Ordered accumulation of one leaf cell (HydroSolver.cpp:476-547 in
+// processing order).  s_r/s_k hold, for this thread, the six incoming
+// contributions and their keys (negative key: none).
+template <int NT>
+RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const double fs[5], double dtdx,
+                          const GasParams& gp, double (*s_r)[NT], long long (*s_k)[NT], int t) {
+    const long long KOWN = own_key * 8 + 6;
     // ---- ordered accumulation (HydroSolver.cpp:476-547 in processing order)
     // Sort the six incoming contributions by key (invalid ones last), then
     // subtract those ordered before this cell, apply the cell's own update
     // and clamps, and subtract the rest.
-    const long long KOWN = ((long long)p * 8 + ic0) * 8 + 6;
     const long long KMAX = 0x7fffffffffffffffLL;
     long long ks[6];
     int ix[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        ks[f] = kin[f] < 0 ? KMAX : kin[f];
+        const long long kf = s_k[f][t];
+        ks[f] = kf < 0 ? KMAX : kf;
         ix[f] = f;
     }
     // Sorting network for 6 elements (12 compare-exchanges).
@@ -351,10 +207,7 @@ __global__ __launch_bounds__(BLOCK) void k_godunov_fp64(MeshDev m, LevelDev lv, 
     for (int k = 0; k < 6; ++k) {
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
-            double v = rin[0][iv];
-#pragma unroll
-            for (int f = 1; f < 6; ++f) v = (ix[k] == f) ? rin[f][iv] : v;
-            rs[k][iv] = v;
+            rs[k][iv] = s_r[ix[k] * 5 + iv][t];
         }
     }
     double u[5];
@@ -401,6 +254,411 @@ __global__ __launch_bounds__(BLOCK) void k_godunov_fp64(MeshDev m, LevelDev lv, 
     for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + ci] = u[iv];
 }
 
+// One thread per cell of the level's octs; non-leaf cells exit.  For each
+// dimension the thread builds the 5-cell line L2 L1 C R1 R2 through its oct
+// and the two neighbour octs, computes its own two face fluxes and the two
+// fluxes its same-level neighbours L1/R1 compute for the shared faces, then
+// adds everything into unew in the reference's processing order.
+template <int RIEMANN, bool MIXED>
+__global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
+                                                      double dx) {
+    using TC = typename std::conditional<MIXED, float, double>::type;
+    // Per-thread reflux payload: the six incoming contributions and their
+    // processing keys, kept in LDS so they can be read back in sorted order.
+    __shared__ double s_r[30][G_BLOCK];
+    __shared__ long long s_k[6][G_BLOCK];
+    const int t = threadIdx.x;
+    const int p = blockIdx.x * G_OCTS + (t % G_OCTS);
+    const int ic0 = t / G_OCTS;
+    if (p >= lv.noct) return;
+    if (!((lv.leaf[p] >> ic0) & 1)) return;
+
+    const Gas<double> gd{gp.gamma, gp.gm1, gp.smalle, gp.smallp, 1.0 / gp.gm1};
+    const Gas<TC> gc{(TC)gp.gamma, (TC)gp.gm1, (TC)gp.smalle, (TC)gp.smallp, (TC)(1.0 / gp.gm1)};
+    const int slope_type = gp.slope_type, interpol_type = gp.interpol_type;
+    const double dt = *d_dt;
+    const double dtdx = dt / dx;
+    const TC dtc = (TC)dt, dxc = (TC)dx;
+    const int noct = lv.noct;
+    const int cC = cell_id(m, lv.grid[p], ic0);
+
+    double uC[5];
+    TC qC[5];
+    load_u(m.uold, m.ncell, cC, uC);
+    {
+        double q[5];
+        ctoprim(uC, q, gd);
+        narrow(q, qC);
+    }
+
+    double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+
+    auto same = [&](int pos, int ic) {
+        Nbr n;
+        n.cell = cell_id(m, lv.grid[pos], ic);
+        n.pos = pos;
+        n.ic0 = ic;
+        n.rec = -1;
+        return n;
+    };
+    auto from_rec = [&](int code) {
+        Nbr n;
+        n.rec = -1 - code;
+        n.cell = lv.rec[n.rec].x;
+        n.pos = -1;
+        n.ic0 = 0;
+        return n;
+    };
+    // Primitive state of a cell as read by the slope / ghost stencils (FP64
+    // ctoprim, then the compute type): a boundary index falls back to
+    // `fallback` (the reference copies the centre cell's state there;
+    // bound_type is never filled).
+    auto prim = [&](int icell, const TC fallback[5], TC q[5]) {
+        if (icell > 0) {
+            double u[5], qd[5];
+            load_u(m.uold, m.ncell, icell, u);
+            ctoprim(u, qd, gd);
+            narrow(qd, q);
+        } else {
+            copy5(fallback, q);
+        }
+    };
+    // Boundary face: own state with the normal momentum flipped, zero slopes
+    // (HydroSolver.cpp:130-145).
+    auto boundary = [&](int d, TC qm[5], TC qp[5]) {
+        double u[5], qd[5];
+        copy5(uC, u);
+#pragma unroll
+        for (int i = 1; i <= 3; ++i)
+            if (i == 1 + d) u[i] = u[i] * -1.0;
+        ctoprim(u, qd, gd);
+        TC q[5];
+        narrow(qd, q);
+        rotate(q, d);
+        TC dq[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+        trace(q, dq, dtc, dxc, qm, qp, gc);
+    };
+
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+        const int b = 1 << d;
+        const int par = (ic0 >> d) & 1;
+        const int A = lv.nbo[(size_t)(2 * d) * noct + p];
+        const int B = lv.nbo[(size_t)(2 * d + 1) * noct + p];
+        const Nbr L1 = par ? same(p, ic0 ^ b) : (A >= 0 ? same(A, ic0 ^ b) : from_rec(A));
+        const Nbr R1 = par ? (B >= 0 ? same(B, ic0 ^ b) : from_rec(B)) : same(p, ic0 ^ b);
+
+        TC qL1[5], qR1[5];
+        prim(L1.cell, qC, qL1);
+        prim(R1.cell, qC, qR1);
+
+        // Own face states (trace pass) and this cell's ghost states as its
+        // neighbours see them (coarse-ghost branch with n = this cell).
+        TC OCm[5], OCp[5], GCm[5], GCp[5];
+        own_trace(qL1, qC, qR1, d, slope_type, dtc, dxc, OCm, OCp, gc);
+        ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, dxc, GCm, GCp, gc);
+
+        // ---- minus face (side 0): ql = ghost qm of L1, qr = own qp
+        {
+            double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            long long K = -1;
+            if (L1.pos >= 0) {
+                if ((lv.leaf[L1.pos] >> L1.ic0) & 1) {
+                    TC qL2[5];
+                    const int L2cell = A >= 0 ? cell_id(m, lv.grid[A], ic0) : lv.rec[-1 - A].x;
+                    prim(L2cell, qL1, qL2);
+                    TC Gm[5], Gp[5];
+                    ghost_trace(qL2, qL1, qC, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+                    solve_face<RIEMANN, MIXED>(Gm, OCp, F, gc, gd);
+                    rotate(F, d);
+                    // L1's own plus-face flux into this cell (reflux).
+                    TC Om[5], Op[5];
+                    double Fin[5];
+                    own_trace(qL2, qL1, qC, d, slope_type, dtc, dxc, Om, Op, gc);
+                    solve_face<RIEMANN, MIXED>(Om, GCp, Fin, gc, gd);
+                    rotate(Fin, d);
+#pragma unroll
+                    for (int iv = 0; iv < 5; ++iv) s_r[(2 * d) * 5 + iv][t] = dtdx * -1.0 * Fin[iv] * 0.125;
+                    K = ((long long)L1.pos * 8 + L1.ic0) * 8 + (2 * d + 1);
+                }
+            } else {
+                const int4 rc = lv.rec[L1.rec];
+                if (rc.w & REC_REFINED_DEV) {
+                } else if (rc.w & REC_BOUNDARY_DEV) {
+                    TC Bm[5], Bp[5];
+                    boundary(d, Bm, Bp);
+                    solve_face<RIEMANN, MIXED>(Bm, OCp, F, gc, gd);
+                    rotate(F, d);
+                } else {
+                    TC qn[5], qnl[5], qnr[5];
+                    prim(rc.x, qC, qn);
+                    prim(rc.y, qn, qnl);
+                    prim(rc.z, qn, qnr);
+                    TC Gm[5], Gp[5];
+                    ghost_trace(qnl, qn, qnr, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+                    solve_face<RIEMANN, MIXED>(Gm, OCp, F, gc, gd);
+                    rotate(F, d);
+                    const int slot = lv.rec_slot[L1.rec] + face_slot(ic0, d);
+#pragma unroll
+                    for (int iv = 0; iv < 5; ++iv) lv.slot_r[(size_t)slot * 5 + iv] = dtdx * 1.0 * F[iv] * 0.125;
+                }
+            }
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) fs[iv] = fs[iv] + 1.0 * F[iv];
+            s_k[2 * d][t] = K;
+        }
+        // ---- plus face (side 1): ql = own qm, qr = ghost qp of R1
+        {
+            double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            long long K = -1;
+            if (R1.pos >= 0) {
+                if ((lv.leaf[R1.pos] >> R1.ic0) & 1) {
+                    TC qR2[5];
+                    const int R2cell = B >= 0 ? cell_id(m, lv.grid[B], ic0) : lv.rec[-1 - B].x;
+                    prim(R2cell, qR1, qR2);
+                    TC Gm[5], Gp[5];
+                    ghost_trace(qC, qR1, qR2, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+                    solve_face<RIEMANN, MIXED>(OCm, Gp, F, gc, gd);
+                    rotate(F, d);
+                    // R1's own minus-face flux into this cell (reflux).
+                    TC Om[5], Op[5];
+                    double Fin[5];
+                    own_trace(qC, qR1, qR2, d, slope_type, dtc, dxc, Om, Op, gc);
+                    solve_face<RIEMANN, MIXED>(GCm, Op, Fin, gc, gd);
+                    rotate(Fin, d);
+#pragma unroll
+                    for (int iv = 0; iv < 5; ++iv) s_r[(2 * d + 1) * 5 + iv][t] = dtdx * 1.0 * Fin[iv] * 0.125;
+                    K = ((long long)R1.pos * 8 + R1.ic0) * 8 + (2 * d);
+                }
+            } else {
+                const int4 rc = lv.rec[R1.rec];
+                if (rc.w & REC_REFINED_DEV) {
+                } else if (rc.w & REC_BOUNDARY_DEV) {
+                    TC Bm[5], Bp[5];
+                    boundary(d, Bm, Bp);
+                    solve_face<RIEMANN, MIXED>(OCm, Bp, F, gc, gd);
+                    rotate(F, d);
+                } else {
+                    TC qn[5], qnl[5], qnr[5];
+                    prim(rc.x, qC, qn);
+                    prim(rc.y, qn, qnl);
+                    prim(rc.z, qn, qnr);
+                    TC Gm[5], Gp[5];
+                    ghost_trace(qnl, qn, qnr, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+                    solve_face<RIEMANN, MIXED>(OCm, Gp, F, gc, gd);
+                    rotate(F, d);
+                    const int slot = lv.rec_slot[R1.rec] + face_slot(ic0, d);
+#pragma unroll
+                    for (int iv = 0; iv < 5; ++iv) lv.slot_r[(size_t)slot * 5 + iv] = dtdx * -1.0 * F[iv] * 0.125;
+                }
+            }
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) fs[iv] = fs[iv] + -1.0 * F[iv];
+            s_k[2 * d + 1][t] = K;
+        }
+    }
+
+    ordered_apply<G_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t);
+}
+
+// ---------------------------------------------------------------------------
+// Oct-in-lanes variant: the eight cells of an oct sit in eight adjacent lanes.
+// Per dimension each cell has one neighbour inside its oct (the sibling, lane
+// ^ (1<<d)) and one across the oct face (X, in the neighbour oct E).  The
+// sibling's primitives, traces and its own flux on the shared face come by
+// cross-lane shuffle; only X and the cell beyond it are loaded.  Per cell:
+// 4 traces and 3 Riemann solves per dimension (the per-cell kernel above
+// needs 6 and 4).  The code is the same for both parities of a cell along d
+// (operands chosen with selects), so lanes never diverge on geometry.
+template <typename T> RH_DEV void shfl5(const T in[5], T out[5], int mask) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) out[i] = __shfl_xor(in[i], mask, 64);
+}
+template <typename T> RH_DEV void sel5(bool c, const T a[5], const T b[5], T out[5]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) out[i] = c ? a[i] : b[i];
+}
+
+constexpr int O_OCTS = 16;
+constexpr int O_BLOCK = O_OCTS * 8;
+
+template <int RIEMANN, bool MIXED>
+__global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
+                                                          double dx) {
+    using TC = typename std::conditional<MIXED, float, double>::type;
+    __shared__ double s_r[30][O_BLOCK];
+    __shared__ long long s_k[6][O_BLOCK];
+    const int t = threadIdx.x;
+    const int ic0 = t & 7;
+    const int praw = blockIdx.x * O_OCTS + (t >> 3);
+    const int noct = lv.noct;
+    // Lanes past the end of the list work on the last oct (valid memory) and
+    // write nothing, so every lane of an oct group takes part in the shuffles.
+    const bool valid = praw < noct;
+    const int p = valid ? praw : noct - 1;
+    const int lm = lv.leaf[p];
+    const bool leaf = valid && ((lm >> ic0) & 1);
+
+    const Gas<double> gd{gp.gamma, gp.gm1, gp.smalle, gp.smallp, 1.0 / gp.gm1};
+    const Gas<TC> gc{(TC)gp.gamma, (TC)gp.gm1, (TC)gp.smalle, (TC)gp.smallp, (TC)(1.0 / gp.gm1)};
+    const int slope_type = gp.slope_type, interpol_type = gp.interpol_type;
+    const double dt = *d_dt;
+    const double dtdx = dt / dx;
+    const TC dtc = (TC)dt, dxc = (TC)dx;
+    const int cC = cell_id(m, lv.grid[p], ic0);
+
+    double uC[5];
+    TC qC[5];
+    load_u(m.uold, m.ncell, cC, uC);
+    {
+        double q[5];
+        ctoprim(uC, q, gd);
+        narrow(q, qC);
+    }
+    double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+
+    auto prim = [&](int icell, const TC fallback[5], TC q[5]) {
+        if (icell > 0) {
+            double u[5], qd[5];
+            load_u(m.uold, m.ncell, icell, u);
+            ctoprim(u, qd, gd);
+            narrow(qd, q);
+        } else {
+            copy5(fallback, q);
+        }
+    };
+
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+        const int b = 1 << d;
+        const bool par = (ic0 >> d) & 1;  // 0: sibling on the plus side, X on the minus side
+        const int icS = ic0 ^ b;
+        const int E = lv.nbo[(size_t)(2 * d + (par ? 1 : 0)) * noct + p];
+        int4 rc = make_int4(0, 0, 0, 0);
+        int Xcell;
+        if (E >= 0) {
+            Xcell = cell_id(m, lv.grid[E], icS);
+        } else {
+            rc = lv.rec[-1 - E];
+            Xcell = rc.x;
+        }
+        TC qS[5], qX[5];
+        shfl5(qC, qS, b);
+        prim(Xcell, qC, qX);
+        TC qL1[5], qR1[5];
+        sel5(par, qS, qX, qL1);
+        sel5(par, qX, qS, qR1);
+        TC OCm[5], OCp[5], GCm[5], GCp[5];
+        own_trace(qL1, qC, qR1, d, slope_type, dtc, dxc, OCm, OCp, gc);
+        ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, dxc, GCm, GCp, gc);
+        TC GSm[5], GSp[5];
+        shfl5(GCm, GSm, b);
+        shfl5(GCp, GSp, b);
+
+        // Internal face: this cell's own flux on the sibling side.
+        const bool sib_leaf = (lm >> icS) & 1;
+        double Fint[5];
+        {
+            TC ql[5], qr[5];
+            sel5(par, GSm, OCm, ql);
+            sel5(par, OCp, GSp, qr);
+            solve_face<RIEMANN, MIXED>(ql, qr, Fint, gc, gd);
+            rotate(Fint, d);
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) Fint[iv] = sib_leaf ? Fint[iv] : 0.0;
+        }
+        double Fsib[5];
+        shfl5(Fint, Fsib, b);  // the sibling's own flux on this cell's side
+
+        // External face (across the oct boundary).
+        double Fext[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double Fin[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        long long Kin = -1;
+        if (E >= 0) {
+            if ((lv.leaf[E] >> icS) & 1) {
+                TC qX2[5];
+                prim(cell_id(m, lv.grid[E], ic0), qX, qX2);
+                TC a5[5], c5[5];
+                sel5(par, qC, qX2, a5);
+                sel5(par, qX2, qC, c5);
+                TC GXm[5], GXp[5], OXm[5], OXp[5];
+                ghost_trace(a5, qX, c5, d, interpol_type, dtc, dxc, GXm, GXp, gc);
+                own_trace(a5, qX, c5, d, slope_type, dtc, dxc, OXm, OXp, gc);
+                TC ql[5], qr[5];
+                sel5(par, OCm, GXm, ql);
+                sel5(par, GXp, OCp, qr);
+                solve_face<RIEMANN, MIXED>(ql, qr, Fext, gc, gd);
+                rotate(Fext, d);
+                sel5(par, GCm, OXm, ql);
+                sel5(par, OXp, GCp, qr);
+                solve_face<RIEMANN, MIXED>(ql, qr, Fin, gc, gd);
+                rotate(Fin, d);
+                Kin = ((long long)E * 8 + icS) * 8 + (par ? 2 * d : 2 * d + 1);
+            }
+        } else if (rc.w & REC_REFINED_DEV) {
+        } else if (rc.w & REC_BOUNDARY_DEV) {
+            double u[5], qd[5];
+            copy5(uC, u);
+#pragma unroll
+            for (int i = 1; i <= 3; ++i)
+                if (i == 1 + d) u[i] = u[i] * -1.0;
+            ctoprim(u, qd, gd);
+            TC qb[5];
+            narrow(qd, qb);
+            rotate(qb, d);
+            TC dq0[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+            TC Bm[5], Bp[5];
+            trace(qb, dq0, dtc, dxc, Bm, Bp, gc);
+            TC ql[5], qr[5];
+            sel5(par, OCm, Bm, ql);
+            sel5(par, Bp, OCp, qr);
+            solve_face<RIEMANN, MIXED>(ql, qr, Fext, gc, gd);
+            rotate(Fext, d);
+        } else {
+            TC qnl[5], qnr[5];
+            prim(rc.y, qX, qnl);
+            prim(rc.z, qX, qnr);
+            TC Gm[5], Gp[5];
+            ghost_trace(qnl, qX, qnr, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+            TC ql[5], qr[5];
+            sel5(par, OCm, Gm, ql);
+            sel5(par, Gp, OCp, qr);
+            solve_face<RIEMANN, MIXED>(ql, qr, Fext, gc, gd);
+            rotate(Fext, d);
+            if (leaf) {
+                const int slot = lv.rec_slot[-1 - E] + face_slot(ic0, d);
+                const double sg = par ? -1.0 : 1.0;
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) lv.slot_r[(size_t)slot * 5 + iv] = dtdx * sg * Fext[iv] * 0.125;
+            }
+        }
+
+        // Own fluxes in face order (minus face, then plus face).
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            const double Fm = par ? Fint[iv] : Fext[iv];
+            const double Fp = par ? Fext[iv] : Fint[iv];
+            fs[iv] = fs[iv] + 1.0 * Fm;
+            fs[iv] = fs[iv] + -1.0 * Fp;
+        }
+        // Incoming contributions: from the sibling (its face toward this
+        // cell: plus face when it is on the minus side) and from X.
+        const int slotS = 2 * d + (par ? 0 : 1);
+        const int slotX = 2 * d + (par ? 1 : 0);
+        const double sgS = par ? -1.0 : 1.0;
+        const double sgX = par ? 1.0 : -1.0;
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            s_r[slotS * 5 + iv][t] = dtdx * sgS * Fsib[iv] * 0.125;
+            s_r[slotX * 5 + iv][t] = dtdx * sgX * Fin[iv] * 0.125;
+        }
+        s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
+        s_k[slotX][t] = Kin;
+    }
+    if (!leaf) return;
+    ordered_apply<O_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t);
+}
+
 // Reflux into coarser leaf cells, one thread per target cell, contributions
 // in processing order.
 __global__ __launch_bounds__(256) void k_coarse_reflux(MeshDev m, LevelDev lv) {
@@ -426,7 +684,7 @@ __global__ __launch_bounds__(64) void k_interpol(int n_octs, const double* __res
     if (o >= n_octs) return;
     const double* U1 = u1 + (size_t)o * 7 * 64;
     double* U2 = u2 + (size_t)o * 8 * 64;
-    const Gas<double> g{gp.gamma, gp.gm1, gp.smalle, gp.smallp};
+    const Gas<double> g{gp.gamma, gp.gm1, gp.smalle, gp.smallp, 1.0 / gp.gm1};
     const int it = gp.interpol_type;
     double q1[7][5];
     for (int i = 0; i < 7; ++i) {
@@ -541,17 +799,35 @@ hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, doubl
 }
 
 hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams& gp, int riemann, int precision,
-                          const double* d_dt, double dx, hipStream_t stream) {
+                          int variant, const double* d_dt, double dx, hipStream_t stream) {
     if (lv.noct == 0) return hipSuccess;
-    if (precision != 0) return hipErrorNotSupported;
-    dim3 grid(blocks_for(lv.noct)), block(BLOCK);
+    if (precision != 0 && precision != 1) return hipErrorNotSupported;
+    const bool mixed = precision == 1;
+    const bool oct = variant == 1;
+    dim3 grid(oct ? (lv.noct + O_OCTS - 1) / O_OCTS : (lv.noct + G_OCTS - 1) / G_OCTS);
+    dim3 block(oct ? O_BLOCK : G_BLOCK);
+#define RH_LAUNCH(R)                                                                                   \
+    do {                                                                                               \
+        if (oct) {                                                                                     \
+            if (mixed)                                                                                 \
+                hipLaunchKernelGGL((k_godunov_oct<R, true>), grid, block, 0, stream, m, lv, gp, d_dt, dx);  \
+            else                                                                                       \
+                hipLaunchKernelGGL((k_godunov_oct<R, false>), grid, block, 0, stream, m, lv, gp, d_dt, dx); \
+        } else {                                                                                       \
+            if (mixed)                                                                                 \
+                hipLaunchKernelGGL((k_godunov<R, true>), grid, block, 0, stream, m, lv, gp, d_dt, dx);      \
+            else                                                                                       \
+                hipLaunchKernelGGL((k_godunov<R, false>), grid, block, 0, stream, m, lv, gp, d_dt, dx);     \
+        }                                                                                              \
+    } while (0)
     switch (riemann) {
-        case 0: hipLaunchKernelGGL(k_godunov_fp64<0>, grid, block, 0, stream, m, lv, gp, d_dt, dx); break;
-        case 1: hipLaunchKernelGGL(k_godunov_fp64<1>, grid, block, 0, stream, m, lv, gp, d_dt, dx); break;
-        case 2: hipLaunchKernelGGL(k_godunov_fp64<2>, grid, block, 0, stream, m, lv, gp, d_dt, dx); break;
-        case 4: hipLaunchKernelGGL(k_godunov_fp64<4>, grid, block, 0, stream, m, lv, gp, d_dt, dx); break;
+        case 0: RH_LAUNCH(0); break;
+        case 1: RH_LAUNCH(1); break;
+        case 2: RH_LAUNCH(2); break;
+        case 4: RH_LAUNCH(4); break;
         default: return hipErrorNotSupported;
     }
+#undef RH_LAUNCH
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || lv.ntgt == 0) return e;
     hipLaunchKernelGGL(k_coarse_reflux, dim3((lv.ntgt + 255) / 256), dim3(256), 0, stream, m, lv);

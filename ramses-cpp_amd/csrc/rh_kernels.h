@@ -51,8 +51,10 @@ hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, doubl
                           double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream);
 int courant_blocks(const LevelDev& lv);
 // Godunov update of one level; dt is read from device memory (*d_dt).
+// variant 0: one thread per cell, octs strided across lanes; variant 1: the
+// eight cells of an oct in adjacent lanes, sharing by cross-lane shuffles.
 hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams& gp, int riemann, int precision,
-                          const double* d_dt, double dx, hipStream_t stream);
+                          int variant, const double* d_dt, double dx, hipStream_t stream);
 hipError_t launch_interpol(int n_octs, const double* d_u1, double* d_u2, const GasParams& gp, int interpol_var,
                            hipStream_t stream);
 

@@ -30,8 +30,13 @@ template <typename T> RH_DEV T smin3(T a, T b, T c) {
     return m;
 }
 template <typename T> RH_DEV T sabs(T a) { return __builtin_fabs(a); }
+// Division and square root: correctly rounded for double (the FP64 form);
+// hardware reciprocal / square root (about 1 ulp) for float, which only the
+// mixed form's FP32 half uses.
 RH_DEV double ssqrt(double a) { return __builtin_sqrt(a); }
-RH_DEV float ssqrt(float a) { return __builtin_sqrtf(a); }
+RH_DEV float ssqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+RH_DEV double dv(double a, double b) { return a / b; }
+RH_DEV float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 template <typename T> RH_DEV void swap_(T& a, T& b) {
     T t = a;
@@ -40,9 +45,13 @@ template <typename T> RH_DEV void swap_(T& a, T& b) {
 }
 
 // Rotate a 5-vector into the frame of dimension d (swap 1 <-> 1+d).
+// Written with value selects so a runtime d never turns q into an indexed
+// (scratch-memory) array.
 template <typename T> RH_DEV void rotate(T q[5], int d) {
-    if (d == 1) swap_(q[1], q[2]);
-    else if (d == 2) swap_(q[1], q[3]);
+    const T a = q[1], b = q[2], c = q[3];
+    q[1] = (d == 1) ? b : ((d == 2) ? c : a);
+    q[2] = (d == 1) ? a : b;
+    q[3] = (d == 2) ? a : c;
 }
 
 // Per-call constants derived from the namelist.
@@ -51,13 +60,14 @@ template <typename T> struct Gas {
     T gm1;       // gamma - 1
     T smalle;    // ((1e-10*1e-10)/gamma)/(gamma-1), ctoprim floor
     T smallp;    // (1e-10*1e-10)/gamma, Riemann pressure floor factor
+    T inv_gm1;   // 1/(gamma-1), used only by the mixed form's FP64 assembly
 };
 
 // Conservative -> primitive (HydroSolver.cpp:613-651, nener=0).
 template <typename T> RH_DEV void ctoprim(const T u[5], T q[5], const Gas<T>& g) {
     T d = smax(u[0], T(1e-10));
     q[0] = d;
-    T oneoverrho = T(1.0) / d;
+    T oneoverrho = dv(T(1.0), d);
     T eken = T(0.0);
     q[1] = u[1] * oneoverrho;
     eken = eken + T(0.5) * q[1] * q[1];
@@ -73,7 +83,7 @@ template <typename T> RH_DEV void ctoprim(const T u[5], T q[5], const Gas<T>& g)
 // Sound speed squared (EquationOfState.cpp:26-51, no barotropic EOS, nener=0).
 template <typename T> RH_DEV T cs2(T density, T pressure, const Gas<T>& g) {
     T c = g.gamma * pressure;
-    return smax(c / smax(density, T(1e-10)), T(1e-20));
+    return smax(dv(c, smax(density, T(1e-10))), T(1e-20));
 }
 
 // TVD slope limiter (SlopeLimiter.cpp:7-60), theta = 1.5.
@@ -84,14 +94,14 @@ template <typename T> RH_DEV T limiter(T ql, T qc, T qr, int slope_type, T nu) {
     if (slope_type == 4 || slope_type == 5) {
         T dlft_s, drgt_s;
         if (slope_type == 4) {
-            dlft_s = T(2.0) / (T(1.0) + nu) * dlft;
-            drgt_s = T(2.0) / (T(1.0) - nu) * drgt;
+            dlft_s = dv(T(2.0), T(1.0) + nu) * dlft;
+            drgt_s = dv(T(2.0), T(1.0) - nu) * drgt;
         } else if (nu >= T(0.0)) {
-            dlft_s = T(2.0) / (nu + T(1e-10)) * dlft;
-            drgt_s = T(2.0) / (T(1.0) - nu) * drgt;
+            dlft_s = dv(T(2.0), nu + T(1e-10)) * dlft;
+            drgt_s = dv(T(2.0), T(1.0) - nu) * drgt;
         } else {
-            dlft_s = T(2.0) / (T(1.0) + nu) * dlft;
-            drgt_s = T(2.0) / (-nu + T(1e-10)) * drgt;
+            dlft_s = dv(T(2.0), T(1.0) + nu) * dlft;
+            drgt_s = dv(T(2.0), -nu + T(1e-10)) * drgt;
         }
         T dcen = T(0.5) * (dlft + drgt);
         T dsgn = (dcen >= T(0.0)) ? T(1.0) : T(-1.0);
@@ -109,7 +119,7 @@ template <typename T> RH_DEV T limiter(T ql, T qc, T qr, int slope_type, T nu) {
     } else if (slope_type == 2 || slope_type == 3) {
         dlim = smin3(T(2.0) * sabs(dlft), T(2.0) * sabs(drgt), sabs(dcen));
     } else if (slope_type == 7) {
-        dlim = T(2.0) * sabs(dlft) * sabs(drgt) / (sabs(dlft) + sabs(drgt));
+        dlim = dv(T(2.0) * sabs(dlft) * sabs(drgt), sabs(dlft) + sabs(drgt));
     } else if (slope_type == 8) {
         dlim = smin3(T(1.5) * sabs(dlft), T(1.5) * sabs(drgt), sabs(dcen));
     } else {
@@ -137,7 +147,7 @@ RH_DEV void trace(const T q[5], const T dq[5], T dt, T dx, T qm[5], T qp[5], con
     T u = q[1];
     T p = smax(q[4], r * T(1e-20));
     T sr0 = -u * dq[0] - dq[1] * r;
-    T su0 = -u * dq[1] - dq[4] / r;
+    T su0 = -u * dq[1] - dv(dq[4], r);
     T c2 = cs2(r, p, g);
     T sp0 = -u * dq[4] - dq[1] * r * c2;
     T hdx = T(0.5) * dx;
@@ -173,11 +183,11 @@ RH_DEV void own_trace(const T ql[5], const T qc[5], const T qr[5], int d, int sl
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) dq[iv] = T(0.0);
     } else {
-        T nu = b[1] * dt / dx;
+        T nu = dv(b[1] * dt, dx);
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
             if ((slope_type == 5 || slope_type == 6) && iv != 0) dq[iv] = T(0.0);
-            else dq[iv] = limiter(a[iv], b[iv], c[iv], slope_type, nu) / dx;
+            else dq[iv] = dv(limiter(a[iv], b[iv], c[iv], slope_type, nu), dx);
         }
     }
     trace(b, dq, dt, dx, qm, qp, g);
@@ -202,7 +212,7 @@ RH_DEV void ghost_trace(const T ql[5], const T qc[5], const T qr[5], int d, int 
     T dx_coarse = T(2.0) * dx;
     T dq[5];
 #pragma unroll
-    for (int iv = 0; iv < 5; ++iv) dq[iv] = ghost_limiter(b[iv] - a[iv], c[iv] - b[iv], interpol_type) / dx_coarse;
+    for (int iv = 0; iv < 5; ++iv) dq[iv] = dv(ghost_limiter(b[iv] - a[iv], c[iv] - b[iv], interpol_type), dx_coarse);
     trace(b, dq, dt, dx_coarse, qm, qp, g);
 }
 
@@ -213,7 +223,11 @@ RH_DEV void boundary_trace(const T u_own[5], bool reflect, int d, T dt, T dx, T 
     T u[5];
 #pragma unroll
     for (int iv = 0; iv < 5; ++iv) u[iv] = u_own[iv];
-    if (reflect) u[1 + d] = u[1 + d] * T(-1.0);
+    if (reflect) {
+#pragma unroll
+        for (int i = 1; i <= 3; ++i)
+            if (i == 1 + d) u[i] = u[i] * T(-1.0);
+    }
     T q[5];
     ctoprim(u, q, g);
     rotate(q, d);
@@ -232,7 +246,7 @@ template <typename T> RH_DEV void prim_to_cons(const T q[5], T u[5], const Gas<T
     u[3] = q[0] * q[3];
     v2 = v2 + q[3] * q[3];
     T e_kin = T(0.5) * q[0] * v2;
-    T e_int = q[4] / g.gm1;
+    T e_int = dv(q[4], g.gm1);
     T e_nonthermal = T(0.0);
     u[4] = e_kin + e_int + e_nonthermal;
 }
@@ -247,7 +261,7 @@ template <typename T> RH_DEV void phys_flux(const T q[5], T f[5], const Gas<T>& 
     v2 = v2 + q[2] * q[2];
     v2 = v2 + q[3] * q[3];
     T e_kin = T(0.5) * rho * v2;
-    T e_int = p / g.gm1;
+    T e_int = dv(p, g.gm1);
     T e_nonthermal = T(0.0);
     T p_tot = p;
     T E = e_kin + e_int + e_nonthermal;
@@ -287,7 +301,7 @@ template <typename T> RH_DEV void riemann_hll(const T ql[5], const T qr[5], T fl
 #pragma unroll
         for (int i = 0; i < 5; ++i) flux[i] = T(0.5) * (fl[i] + fr[i]);
     } else {
-        T inv_srs = T(1.0) / (sr - sl);
+        T inv_srs = dv(T(1.0), sr - sl);
 #pragma unroll
         for (int i = 0; i < 5; ++i) flux[i] = (sr * fl[i] - sl * fr[i] + sr * sl * (ur[i] - ul[i])) * inv_srs;
     }
@@ -326,9 +340,9 @@ template <typename T> RH_DEV void riemann_hllc(const T ql[5], const T qr[5], T f
         for (int i = 0; i < 5; ++i) flux[i] = T(0.5) * (fl[i] + fr[i]);
         return;
     }
-    T ustar = (rcr * ur + rcl * ul + (pl - pr)) / div;
+    T ustar = dv(rcr * ur + rcl * ul + (pl - pr), div);
     ustar = smax(T(-1e6), smin(T(1e6), ustar));
-    T pstar = (rcr * pl + rcl * pr + rcl * rcr * (ul - ur)) / div;
+    T pstar = dv(rcr * pl + rcl * pr + rcl * rcr * (ul - ur), div);
     pstar = smax(pstar, T(0.0));
     // One side of the contact: the left star state for ustar > 0, otherwise
     // the right one; both branches share this code with mirrored inputs.
@@ -339,7 +353,7 @@ template <typename T> RH_DEV void riemann_hllc(const T ql[5], const T qr[5], T f
     T pk = left ? pl : pr;
     T sk = left ? sl : sr;
     T den = left ? smin(sl - ustar, T(-1e-10)) : smax(sr - ustar, T(1e-10));
-    T rstar = rk * (sk - uk) / den;
+    T rstar = dv(rk * (sk - uk), den);
     T qs[5];
     qs[0] = rstar;
     qs[1] = ustar;
@@ -351,7 +365,7 @@ template <typename T> RH_DEV void riemann_hllc(const T ql[5], const T qr[5], T f
     phys_flux(qs_src, fk, g);
     prim_to_cons(qs_src, uk_v, g);
     prim_to_cons(qs, us_v, g);
-    us_v[4] = ((sk - uk) * uk_v[4] - pk * uk + pstar * ustar) / den;
+    us_v[4] = dv((sk - uk) * uk_v[4] - pk * uk + pstar * ustar, den);
 #pragma unroll
     for (int i = 0; i < 5; ++i) flux[i] = fk[i] + sk * (us_v[i] - uk_v[i]);
 }
@@ -370,14 +384,14 @@ template <typename T> RH_DEV void riemann_acoustic(const T ql[5], const T qr[5],
     T cr = ssqrt(cs2(rr, qr[4], g));
     T wl = cl * rl;
     T wr = cr * rr;
-    T pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr + T(1e-20));
-    T ustar = ((wr * ur + wl * ul) + (pl - pr)) / (wl + wr + T(1e-20));
+    T pstar = dv((wr * pl + wl * pr) + wl * wr * (ul - ur), wl + wr + T(1e-20));
+    T ustar = dv((wr * ur + wl * ul) + (pl - pr), wl + wr + T(1e-20));
     T sgnm = (ustar >= T(0.0)) ? T(1.0) : T(-1.0);
     bool lft = sgnm == T(1.0);
     T ro = lft ? rl : rr, uo = lft ? ul : ur, po = lft ? pl : pr, co = lft ? cl : cr;
-    T rstar = ro + (pstar - po) / smax(co * co, T(1e-20));
+    T rstar = ro + dv(pstar - po, smax(co * co, T(1e-20)));
     rstar = smax(rstar, smallr);
-    T cstar = ssqrt(smax(g.gamma * pstar / rstar, T(1e-20)));
+    T cstar = ssqrt(smax(dv(g.gamma * pstar, rstar), T(1e-20)));
     cstar = smax(cstar, smallc);
     T spout = co - sgnm * uo;
     T spin = cstar - sgnm * ustar;
@@ -393,7 +407,7 @@ template <typename T> RH_DEV void riemann_acoustic(const T ql[5], const T qr[5],
     } else if (spin >= T(0.0)) {
         rg = rstar; ug = ustar; pg = pstar;
     } else {
-        T frac = spout / (spout - spin + T(1e-20));
+        T frac = dv(spout, spout - spin + T(1e-20));
         rg = frac * rstar + (T(1.0) - frac) * ro;
         ug = frac * ustar + (T(1.0) - frac) * uo;
         pg = frac * pstar + (T(1.0) - frac) * po;
@@ -414,6 +428,188 @@ RH_DEV void riemann(const T ql[5], const T qr[5], T flux[5], const Gas<T>& g) {
     else if constexpr (RIEMANN == 1) riemann_hll(ql, qr, flux, g);
     else if constexpr (RIEMANN == 4) riemann_acoustic(ql, qr, flux, g);
     else riemann_llf(ql, qr, flux, g);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed form: the wave structure (sound speeds, wave speeds, star states) is
+// solved in FP32 from FP32 face states; the conservative states, physical
+// fluxes and the flux assembly F_K + S_K (U*_K - U_K) are FP64.
+
+RH_DEV void widen(const float a[5], double b[5]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) b[i] = (double)a[i];
+}
+
+RH_DEV void prim_to_cons_m(const double q[5], double u[5], const Gas<double>& g) {
+    u[0] = q[0];
+    u[1] = q[0] * q[1];
+    u[2] = q[0] * q[2];
+    u[3] = q[0] * q[3];
+    u[4] = 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + q[4] * g.inv_gm1;
+}
+
+RH_DEV void phys_flux_m(const double q[5], double f[5], const Gas<double>& g) {
+    const double mu = q[0] * q[1];
+    const double E = 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + q[4] * g.inv_gm1;
+    f[0] = mu;
+    f[1] = mu * q[1] + q[4];
+    f[2] = mu * q[2];
+    f[3] = mu * q[3];
+    f[4] = (E + q[4]) * q[1];
+}
+
+RH_DEV void riemann_hllc_m(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
+                           const Gas<double>& gd) {
+    const float rl = smax(ql[0], 1e-10f), ul = ql[1], pl = smax(ql[4], rl * gf.smallp);
+    const float rr = smax(qr[0], 1e-10f), ur = qr[1], pr = smax(qr[4], rr * gf.smallp);
+    const float cl = ssqrt(cs2(rl, ql[4], gf));
+    const float cr = ssqrt(cs2(rr, qr[4], gf));
+    const float cmax = smax(cl, cr);
+    const float sl = smin(ul, ur) - cmax;
+    const float sr = smax(ul, ur) + cmax;
+    double qd[5];
+    if (sl > 0.0f) {
+        widen(ql, qd);
+        phys_flux_m(qd, flux, gd);
+        return;
+    }
+    if (sr < 0.0f) {
+        widen(qr, qd);
+        phys_flux_m(qd, flux, gd);
+        return;
+    }
+    const float rcl = rl * (ul - sl);
+    const float rcr = rr * (sr - ur);
+    const float div = rcr + rcl;
+    if (div < 1e-5f) {
+        double fl[5], fr[5];
+        widen(ql, qd);
+        phys_flux_m(qd, fl, gd);
+        widen(qr, qd);
+        phys_flux_m(qd, fr, gd);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) flux[i] = 0.5 * (fl[i] + fr[i]);
+        return;
+    }
+    const float idiv = __builtin_amdgcn_rcpf(div);
+    float ustar = (rcr * ur + rcl * ul + (pl - pr)) * idiv;
+    ustar = smax(-1e6f, smin(1e6f, ustar));
+    float pstar = (rcr * pl + rcl * pr + rcl * rcr * (ul - ur)) * idiv;
+    pstar = smax(pstar, 0.0f);
+    const bool left = ustar > 0.0f;
+    const float* qk = left ? ql : qr;
+    const float rk = left ? rl : rr, uk = left ? ul : ur, pk = left ? pl : pr, sk = left ? sl : sr;
+    const float den = left ? smin(sl - ustar, -1e-10f) : smax(sr - ustar, 1e-10f);
+    const float rstar = rk * (sk - uk) * __builtin_amdgcn_rcpf(den);
+    double qkd[5], qs[5], fk[5], ukv[5], usv[5];
+    widen(qk, qkd);
+    qs[0] = rstar;
+    qs[1] = ustar;
+    qs[2] = qkd[2];
+    qs[3] = qkd[3];
+    qs[4] = (double)smax(pstar, rstar * gf.smallp);
+    phys_flux_m(qkd, fk, gd);
+    prim_to_cons_m(qkd, ukv, gd);
+    prim_to_cons_m(qs, usv, gd);
+    const double skd = sk, ukd = uk, dend = den;
+    usv[4] = ((skd - ukd) * ukv[4] - (double)pk * ukd + (double)pstar * (double)ustar) / dend;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) flux[i] = fk[i] + skd * (usv[i] - ukv[i]);
+}
+
+RH_DEV void riemann_hll_m(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
+                          const Gas<double>& gd) {
+    const float al = ssqrt(cs2(ql[0], ql[4], gf));
+    const float ar = ssqrt(cs2(qr[0], qr[4], gf));
+    const float sl = smin(0.0f, smin(ql[1] - al, qr[1] - ar));
+    const float sr = smax(0.0f, smax(ql[1] + al, qr[1] + ar));
+    double a[5], b[5], fl[5], fr[5], ul[5], ur[5];
+    widen(ql, a);
+    widen(qr, b);
+    phys_flux_m(a, fl, gd);
+    phys_flux_m(b, fr, gd);
+    if (sr - sl < 1e-20f) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) flux[i] = 0.5 * (fl[i] + fr[i]);
+        return;
+    }
+    prim_to_cons_m(a, ul, gd);
+    prim_to_cons_m(b, ur, gd);
+    const double sld = sl, srd = sr;
+    const double inv = 1.0 / (srd - sld);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) flux[i] = (srd * fl[i] - sld * fr[i] + srd * sld * (ur[i] - ul[i])) * inv;
+}
+
+RH_DEV void riemann_llf_m(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
+                          const Gas<double>& gd) {
+    const float al = ssqrt(cs2(ql[0], ql[4], gf));
+    const float ar = ssqrt(cs2(qr[0], qr[4], gf));
+    const double a_max = (double)smax(sabs(ql[1]) + al, sabs(qr[1]) + ar);
+    double a[5], b[5], fl[5], fr[5], ul[5], ur[5];
+    widen(ql, a);
+    widen(qr, b);
+    phys_flux_m(a, fl, gd);
+    phys_flux_m(b, fr, gd);
+    prim_to_cons_m(a, ul, gd);
+    prim_to_cons_m(b, ur, gd);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) flux[i] = 0.5 * (fl[i] + fr[i]) - 0.5 * a_max * (ur[i] - ul[i]);
+}
+
+RH_DEV void riemann_acoustic_m(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
+                               const Gas<double>& gd) {
+    // The sampled Godunov state is found in FP32; its flux is assembled in FP64.
+    float qg[5];
+    {
+        const float rl = smax(ql[0], 1e-10f), ul = ql[1], pl = smax(ql[4], rl * gf.smallp);
+        const float rr = smax(qr[0], 1e-10f), ur = qr[1], pr = smax(qr[4], rr * gf.smallp);
+        const float cl = ssqrt(cs2(rl, ql[4], gf)), cr = ssqrt(cs2(rr, qr[4], gf));
+        const float wl = cl * rl, wr = cr * rr;
+        const float iw = __builtin_amdgcn_rcpf(wl + wr + 1e-20f);
+        const float pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) * iw;
+        const float ustar = ((wr * ur + wl * ul) + (pl - pr)) * iw;
+        const float sgnm = (ustar >= 0.0f) ? 1.0f : -1.0f;
+        const bool lft = sgnm == 1.0f;
+        const float ro = lft ? rl : rr, uo = lft ? ul : ur, po = lft ? pl : pr, co = lft ? cl : cr;
+        float rstar = smax(ro + dv(pstar - po, smax(co * co, 1e-20f)), 1e-10f);
+        float cstar = smax(ssqrt(smax(dv(gf.gamma * pstar, rstar), 1e-20f)), 1e-10f);
+        float spout = co - sgnm * uo;
+        float spin = cstar - sgnm * ustar;
+        float ushock = smax(0.5f * (spin + spout), -sgnm * ustar);
+        if (pstar >= po) {
+            spout = ushock;
+            spin = spout;
+        }
+        float rg, ug, pg;
+        if (spout < 0.0f) {
+            rg = ro; ug = uo; pg = po;
+        } else if (spin >= 0.0f) {
+            rg = rstar; ug = ustar; pg = pstar;
+        } else {
+            const float frac = dv(spout, spout - spin + 1e-20f);
+            rg = frac * rstar + (1.0f - frac) * ro;
+            ug = frac * ustar + (1.0f - frac) * uo;
+            pg = frac * pstar + (1.0f - frac) * po;
+        }
+        qg[0] = rg;
+        qg[1] = ug;
+        qg[2] = lft ? ql[2] : qr[2];
+        qg[3] = lft ? ql[3] : qr[3];
+        qg[4] = smax(pg, rg * gf.smallp);
+    }
+    double q[5];
+    widen(qg, q);
+    phys_flux_m(q, flux, gd);
+}
+
+template <int RIEMANN>
+RH_DEV void riemann_mixed(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
+                          const Gas<double>& gd) {
+    if constexpr (RIEMANN == 2) riemann_hllc_m(ql, qr, flux, gf, gd);
+    else if constexpr (RIEMANN == 1) riemann_hll_m(ql, qr, flux, gf, gd);
+    else if constexpr (RIEMANN == 4) riemann_acoustic_m(ql, qr, flux, gf, gd);
+    else riemann_llf_m(ql, qr, flux, gf, gd);
 }
 
 }  // namespace rh

@@ -92,6 +92,9 @@ def lib() -> C.CDLL:
         L.rh_timers_read.argtypes = [C.c_void_p, C.c_int, _DP, C.POINTER(C.c_longlong)]
         L.rh_timers_reset.argtypes = [C.c_void_p]
         L.rh_level_leaf_cells.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
+        L.rh_set_kernel_variant.argtypes = [C.c_void_p, C.c_int]
+        L.rh_get_kernel_variant.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.rh_autotune.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int), _DP]
         _lib = L
     return _lib
 
@@ -100,7 +103,7 @@ EXPORTED = [
     "rh_create", "rh_destroy", "rh_bind", "rh_set_unew", "rh_godunov_fine", "rh_set_uold", "rh_courant",
     "rh_interpol_hydro", "rh_sync", "rh_last_error", "rh_run_uniform", "rh_uniform_ngrid", "rh_build_uniform",
     "rh_init_regions", "rh_init_lognormal", "rh_timers_enable", "rh_timers_read", "rh_timers_reset",
-    "rh_level_leaf_cells",
+    "rh_level_leaf_cells", "rh_set_kernel_variant", "rh_get_kernel_variant", "rh_autotune",
 ]
 
 
@@ -194,6 +197,7 @@ def uniform_mesh(levelmin: int, nx: int = 1, ny: int = 1, nz: int = 1, ngridmax:
                             _ip(a["numbl"]), _dp(a["xg"]))
     if ng < 0:
         raise RuntimeError(f"rh_build_uniform failed ({ng})")
+    m.headf = int(ng) + 1 if ng < ngridmax else 0  # first grid of the free list
     return m
 
 
@@ -245,7 +249,7 @@ class Solver:
     """One rh_ctx bound to one Mesh (the HydroSolver of the reference)."""
 
     def __init__(self, gamma=1.4, slope_type=1, riemann="hllc", interpol_type=1, interpol_var=0, boxlen=1.0,
-                 precision=RH_PREC_FP64, device=0):
+                 precision=RH_PREC_FP64, device=0, variant=None):
         self.params = RhParams(gamma, slope_type, riemann if isinstance(riemann, int) else resolve_riemann(riemann),
                                interpol_type, interpol_var, 5, 0, 0, boxlen, precision, device)
         self._ctx = C.c_void_p()
@@ -255,6 +259,8 @@ class Solver:
         self.mesh = None
         self._rhm = None
         self._f = None
+        if variant is not None:
+            self._chk(lib().rh_set_kernel_variant(self._ctx, variant))
 
     def _chk(self, st):
         if st != RH_OK:
@@ -308,6 +314,17 @@ class Solver:
 
     def timers_reset(self):
         self._chk(lib().rh_timers_reset(self._ctx))
+
+    def autotune(self, ilevel, dt, dx, reps=3):
+        best = C.c_int()
+        ms = np.zeros(2, np.float64)
+        self._chk(lib().rh_autotune(self._ctx, ilevel, dt, dx, reps, C.byref(best), _dp(ms)))
+        return best.value, ms
+
+    def variant(self) -> int:
+        v = C.c_int()
+        self._chk(lib().rh_get_kernel_variant(self._ctx, C.byref(v)))
+        return v.value
 
     def leaf_cells(self, ilevel) -> int:
         n = C.c_longlong()

@@ -1,0 +1,114 @@
+"""GPU parity on AMR meshes: levels with coarse-fine faces (ghost states
+interpolated from coarser leaf cells, reflux into those cells in processing
+order through the coarse-target lists) and levels whose cells have refined
+neighbours (zero flux) or are refined themselves (skipped)."""
+import numpy as np
+import pytest
+
+import oracle
+import rhydro
+from amr_meshes import refined_mesh
+from helpers import assert_same, clone
+
+pytestmark = pytest.mark.gpu
+
+SPHERES = ((0.4, 0.5, 0.6, 0.3), (0.45, 0.5, 0.55, 0.15))
+
+
+def _sequence(api, levels, dts):
+    """Finer levels first, as the recursion of amr_step does."""
+    for L in levels:
+        api.set_unew(L)
+    for L, dt in zip(levels, dts):
+        api.godunov_fine(L, dt, 1.0 / 2 ** L)
+    for L in levels:
+        api.set_uold(L)
+
+
+class _OracleApi:
+    def __init__(self, m, p):
+        self.om, self.p = oracle.OracleMesh(m), p
+
+    def set_unew(self, L):
+        self.om.set_unew(L)
+
+    def set_uold(self, L):
+        self.om.set_uold(L)
+
+    def godunov_fine(self, L, dt, dx):
+        self.om.godunov_fine(self.p, L, dt, dx)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("riemann", ["llf", "hllc"])
+@pytest.mark.parametrize("mode", [rhydro.RH_DEVICE_RESIDENT, rhydro.RH_HOST_COHERENT])
+def test_two_level_refinement(riemann, mode, variant):
+    m0 = refined_mesh(4, SPHERES, riemann=rhydro.resolve_riemann(riemann))
+    levels = [6, 5, 4]
+    dts = [2e-3, 4e-3, 8e-3]
+    print(f"case amr {riemann} mode={mode} octs={m0.numbl.tolist()}", flush=True)
+    mg, mo = clone(m0), clone(m0)
+    s = rhydro.Solver(gamma=1.4, slope_type=1, riemann=riemann, variant=variant)
+    s.bind(mg, mode)
+    _sequence(s, levels, dts)
+    if mode == rhydro.RH_DEVICE_RESIDENT:
+        s.sync_to_host()
+    _sequence(_OracleApi(mo, oracle.params(1.4, 1, rhydro.resolve_riemann(riemann))), levels, dts)
+    assert_same(mg.unew, mo.unew, "unew")
+    assert_same(mg.uold, mo.uold, "uold")
+    s.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("slope_type,interpol_type", [(2, 1), (1, 2), (8, 3), (0, 0)])
+def test_single_call_on_each_amr_level(slope_type, interpol_type, variant):
+    m0 = refined_mesh(4, SPHERES, seed=11, interpol_type=interpol_type)
+    for L in (4, 5, 6):
+        print(f"case amr level {L} slope {slope_type} interpol {interpol_type}", flush=True)
+        mg, mo = clone(m0), clone(m0)
+        s = rhydro.Solver(gamma=1.4, slope_type=slope_type, riemann="hllc", interpol_type=interpol_type,
+                          variant=variant)
+        s.bind(mg)
+        s.set_unew(L)
+        s.godunov_fine(L, 3e-3, 1.0 / 2 ** L)
+        s.sync_to_host()
+        om = oracle.OracleMesh(mo)
+        om.set_unew(L)
+        om.godunov_fine(oracle.params(1.4, slope_type, 2, interpol_type), L, 3e-3, 1.0 / 2 ** L)
+        assert_same(mg.unew, mo.unew, f"unew level {L}")
+        s.close()
+
+
+def test_courant_on_amr_levels_skips_refined_cells():
+    m0 = refined_mesh(4, SPHERES, seed=3)
+    s = rhydro.Solver(gamma=1.4)
+    s.bind(m0)
+    om = oracle.OracleMesh(clone(m0))
+    for L in (4, 5, 6):
+        print(f"case amr courant level {L}", flush=True)
+        assert s.courant(L, 1.0 / 2 ** L, 1.4, 0.8) == om.courant(L, 1.0 / 2 ** L, 1.4, 0.8)
+    s.close()
+
+
+def test_rebind_after_refinement():
+    """A topology change bumps the epoch; the tables are rebuilt."""
+    m = refined_mesh(4, SPHERES[:1], extra_grids=4000, seed=5, extra_levels=1)
+    s = rhydro.Solver(gamma=1.4, riemann="hllc")
+    s.bind(m, rhydro.RH_HOST_COHERENT)
+    s.set_unew(5)
+    s.godunov_fine(5, 1e-3, 1 / 32)
+    from amr_meshes import cell_centres
+
+    cells, cen = cell_centres(m, 5)
+    inside = ((cen - np.array([0.4, 0.5, 0.6])) ** 2).sum(1) < 0.1 ** 2
+    oracle.refine(m, oracle.params(1.4, 1, 2), 5, cells[inside & (m.son[cells - 1] == 0)])
+    mo = clone(m)
+    s.bind(m, rhydro.RH_HOST_COHERENT)
+    print("case amr rebind", flush=True)
+    for api in (s, _OracleApi(mo, oracle.params(1.4, 1, 2))):
+        api.set_unew(6)
+        api.set_unew(5)
+        api.godunov_fine(6, 5e-4, 1 / 64)
+        api.godunov_fine(5, 1e-3, 1 / 32)
+    assert_same(m.unew, mo.unew, "unew after rebind")
+    s.close()

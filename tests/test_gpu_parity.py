@@ -39,11 +39,12 @@ def _compare_steps(m0, solver_kw, level, nsteps, cf=0.8, mode=rhydro.RH_DEVICE_R
     s.close()
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("riemann", RIEMANNS)
 @pytest.mark.parametrize("slope_type", range(9))
-def test_step_riemann_slope(riemann, slope_type):
+def test_step_riemann_slope(riemann, slope_type, variant):
     m = _random_mesh(4, seed=1000 + slope_type)
-    _compare_steps(m, dict(gamma=5.0 / 3.0, slope_type=slope_type, riemann=riemann), 4, 2)
+    _compare_steps(m, dict(gamma=5.0 / 3.0, slope_type=slope_type, riemann=riemann, variant=variant), 4, 2)
 
 
 @pytest.mark.parametrize("interpol_type", range(5))
@@ -52,11 +53,12 @@ def test_step_interpol_type(interpol_type):
     _compare_steps(m, dict(gamma=1.4, slope_type=2, riemann="hllc", interpol_type=interpol_type), 4, 2)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("level", [2, 3, 5])
-def test_small_and_degenerate_levels(level):
+def test_small_and_degenerate_levels(level, variant):
     # level 2 is a single oct whose every neighbour is itself (periodic).
     m = _random_mesh(level, seed=5 + level)
-    _compare_steps(m, dict(gamma=1.4, slope_type=1, riemann="hllc"), level, 3)
+    _compare_steps(m, dict(gamma=1.4, slope_type=1, riemann="hllc", variant=variant), level, 3)
 
 
 def test_multi_coarse_cells():
@@ -78,15 +80,16 @@ def test_sedov_32_trajectory():
     _compare_steps(m, dict(gamma=1.4, slope_type=1, riemann="hllc"), level, 10)
 
 
-def test_sedov_64_run_uniform_20_steps():
+@pytest.mark.parametrize("variant", [0, 1])
+def test_sedov_64_run_uniform_20_steps(variant):
     """Config 1 at its stated size (64^3, reference level 7), 20 steps
     through rh_run_uniform, against the oracle step by step."""
     level = 7
     m = rhydro.uniform_mesh(level)
     rhydro.init_regions(m, rhydro.sedov_regions(level), 1.4)
     mg, mo = clone(m), clone(m)
-    print("case sedov64 run_uniform 20 steps", flush=True)
-    s = rhydro.Solver(gamma=1.4, slope_type=1, riemann="hllc")
+    print(f"case sedov64 run_uniform 20 steps variant {variant}", flush=True)
+    s = rhydro.Solver(gamma=1.4, slope_type=1, riemann="hllc", variant=variant)
     s.bind(mg)
     dx = 1.0 / 2 ** level
     dts_g = s.run_uniform(level, dx, 0.8, 20)
@@ -174,4 +177,24 @@ def test_errors_are_reported():
         s.godunov_fine(0, 1e-3, 1e-2)  # coarse level
     with pytest.raises(rhydro.RhError):
         s.godunov_fine(7, 1e-3, 1e-2)  # beyond nlevelmax
+    s.close()
+
+
+def test_autotune_keeps_state_and_results():
+    level = 5
+    m = _random_mesh(level, seed=42)
+    mo = clone(m)
+    s = rhydro.Solver(gamma=5.0 / 3.0, slope_type=1, riemann="hllc")
+    s.bind(m)
+    print("case autotune", flush=True)
+    best, ms = s.autotune(level, 1e-3, 1 / 32, reps=2)
+    assert best in (0, 1) and s.variant() == best and (ms > 0).all()
+    s.sync_to_host()
+    assert_same(m.unew, mo.unew, "unew restored")
+    om = oracle.OracleMesh(mo)
+    p = oracle_params(s)
+    for _ in range(2):
+        assert step_gpu(s, level, 1 / 32, 0.8) == step_oracle(om, p, level, 1 / 32, 0.8)
+    s.sync_to_host()
+    assert_same(m.uold, mo.uold, "uold after tuned steps")
     s.close()

@@ -1,0 +1,83 @@
+"""The oracle against facts recorded from running the reference itself on the
+Sedov blast of benchmark config 1 (64^3 cells = reference level 7; E=1 as
+pressure in the 8 centre cells; HLLC, MinMod, CFL 0.8).  The values were
+logged by a run of the reference's own binary (the survey's probe): they pin
+the composite godunov_fine semantics -- ghost branch on every face, refluxing
+after the clamp in processing order, the Courant pressure cap -- because any
+slip in those changes them.
+
+Recorded values: step 1 dt = 1.760668e-4 and 7 cells with rho < 0 afterwards
+(rho_min = -0.30, largest velocity component 3.692e11); step 2
+dt = 5.642013e-15; after 20 steps no NaN/Inf and rho in [1.0e-10, 1.67e3].
+"""
+import numpy as np
+import pytest
+
+import oracle
+import rhydro
+
+LEVEL = 7
+
+
+@pytest.fixture(scope="module")
+def sedov_run():
+    m = rhydro.uniform_mesh(LEVEL)
+    rhydro.init_regions(m, rhydro.sedov_regions(LEVEL), 1.4)
+    om = oracle.OracleMesh(m)
+    p = oracle.params(1.4, 1, 2)
+    dx = 1.0 / 2 ** LEVEL
+    cells = m.cells_of_level(LEVEL)
+    hist = []
+    for _ in range(20):
+        dt = om.courant(LEVEL, dx, 1.4, 0.8)
+        om.set_unew(LEVEL)
+        om.godunov_fine(p, LEVEL, dt, dx)
+        om.set_uold(LEVEL)
+        u = m.state("uold")[:, cells]
+        hist.append((dt, u.copy()))
+    return hist
+
+
+def test_initial_state_has_eight_hot_cells():
+    m = rhydro.uniform_mesh(LEVEL)
+    rhydro.init_regions(m, rhydro.sedov_regions(LEVEL), 1.4)
+    E = m.state("uold")[4, m.cells_of_level(LEVEL)]
+    hot = E > 1.0
+    assert hot.sum() == 8
+    h = 1.0 / 64
+    assert np.isclose(E[hot].sum() * h ** 3, 1.0, rtol=1e-12)
+
+
+def test_step1(sedov_run):
+    dt, u = sedov_run[0]
+    assert f"{dt:.6e}" == "1.760668e-04"
+    rho = u[0]
+    assert int((rho < 0).sum()) == 7
+    assert round(rho.min(), 2) == -0.30
+    v = u[1:4] / rho
+    assert f"{np.abs(v).max():.3e}" == "3.692e+11"
+
+
+def test_step2_dt(sedov_run):
+    assert f"{sedov_run[1][0]:.6e}" == "5.642013e-15"
+
+
+def test_twenty_steps(sedov_run):
+    dts = np.array([h[0] for h in sedov_run])
+    u = sedov_run[-1][1]
+    assert np.isfinite(u).all() and np.isfinite(dts).all()
+    assert f"{u[0].min():.1e}" == "1.0e-10"
+    assert f"{u[0].max():.3g}" == "1.67e+03"
+    # dt oscillates between ~1e-19 and ~1e-5 (non-physical runaway)
+    assert dts.min() < 1e-18 and dts.max() > 5e-6
+
+
+def test_state_at_rest_stays_unchanged():
+    level = 4
+    m = rhydro.uniform_mesh(level)
+    rhydro.init_regions(m, [dict(type="square", length_x=10, length_y=10, length_z=10, d=1.0, p=1.0)], 1.4)
+    before = m.uold.copy()
+    om = oracle.OracleMesh(m)
+    om.run_uniform(oracle.params(1.4, 2, 2), level, 1 / 16, 0.8, 2)
+    cells = m.cells_of_level(level)
+    assert (m.state("uold")[:, cells] == before.reshape(5, -1)[:, cells]).all()
