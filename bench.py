@@ -27,6 +27,16 @@ METRIC = "hydro cell-updates/s on 1 MI355X, FP64 vs mixed+batched; % of HBM roof
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 80  # read uold (5 x FP64) + write unew (5 x FP64)
 
+# HBM bytes per launch of the Godunov kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE and --pmc WRITE_SIZE runs of this script (each counter scaled by
+# what it reads for k_set_unew, a copy of known size, in the same run); keyed
+# by (level, riemann, slope_type, kernel variant, form).  Counters cannot be
+# read while timing, so only profiled workloads report a figure.
+PROFILED_TRAFFIC = {
+    (9, "hllc", 1, 1, "fp64"): 5.063e9,
+    (9, "hllc", 1, 1, "mixed"): 5.047e9,
+}
+
 
 def cpu_model() -> str:
     try:
@@ -143,6 +153,8 @@ def main() -> int:
             "kernel_ms": mixed["kernel_ms"],
             "autotune": mixed["autotune"],
             "roofline_frac": mixed["achieved"] / HBM_PEAK_GBS,
+            "traffic": PROFILED_TRAFFIC.get((level, args.riemann, args.slope, mixed["autotune"]["chosen_variant"],
+                                             "mixed")),
             "speedup_vs_fp64": fp64["wall"] / mixed["wall"],
             "error_vs_fp64_one_step": err1,
             "error_vs_fp64_after_steps": errn,
@@ -217,7 +229,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": PROFILED_TRAFFIC.get((level, args.riemann, args.slope, fp64["autotune"]["chosen_variant"], "fp64")),
         },
         "cpu_baseline": cpu,
     }
