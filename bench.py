@@ -49,6 +49,83 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def measure_ensemble(args, rhydro, np):
+    """The batched form on the shock-tube ensemble: `boxes` independent
+    periodic boxes of 16-64 cells a side, each with its own level, cell width
+    and time step, advanced together (rh_run_ensemble), in the mixed and FP64
+    forms; then the same boxes one after another through rh_run_uniform (the
+    unbatched alternative), whose final states must equal the batched ones."""
+    gamma, cf = 1.4, 0.8
+    boxes = rhydro.shock_tube_ensemble(args.boxes, seed=args.seed, gamma=gamma)
+    meshes = [b.mesh for b in boxes]
+    levels = [b.level for b in boxes]
+    dxs = [b.dx for b in boxes]
+    u0 = [m.uold.copy() for m in meshes]
+    sides = sorted(b.nx * 2 ** (b.level - 1) for b in boxes)
+    warm = max(args.warmup, 1)
+    out = {}
+    finals = {}
+    for tag, prec in (("mixed", rhydro.RH_PREC_MIXED), ("fp64", rhydro.RH_PREC_FP64)):
+        for m, u in zip(meshes, u0):
+            m.uold[:] = u
+            m.unew[:] = u
+        s = rhydro.Solver(gamma=gamma, slope_type=1, riemann="hllc", precision=prec)
+        s.bind_ensemble(meshes, rhydro.RH_DEVICE_RESIDENT)
+        nleaf = s.ensemble_leaf_cells(levels)
+        s.run_ensemble(levels, dxs, cf, warm)
+        s.timers(True)
+        s.timers_reset()
+        t0 = time.perf_counter()
+        s.run_ensemble(levels, dxs, cf, args.steps)  # synchronises before returning
+        wall = time.perf_counter() - t0
+        k = {name: s.timer(tk) for name, tk in (("godunov", rhydro.TK_GODUNOV), ("courant", rhydro.TK_COURANT),
+                                                ("set_unew", rhydro.TK_SET_UNEW), ("set_uold", rhydro.TK_SET_UOLD))}
+        s.timers(False)
+        s.sync_to_host()
+        s.close()
+        finals[tag] = [m.uold.copy() for m in meshes]
+        god_s = k["godunov"][0] / max(k["godunov"][1], 1) / 1e3
+        out[tag] = {
+            "value": nleaf * args.steps / wall,
+            "ms_per_step": wall / args.steps * 1e3,
+            "kernel_ms": {n: v[0] / max(v[1], 1) for n, v in k.items()},
+            "roofline_frac": BYTES_PER_CELL_UPDATE * nleaf / god_s / 1e9 / HBM_PEAK_GBS,
+        }
+        print(f"[bench] ensemble {tag}: {nleaf} cells, {args.steps} steps in {wall:.4f}s, godunov "
+              f"{god_s * 1e3:.3f} ms", file=sys.stderr, flush=True)
+    # Unbatched: every box alone, mixed form, same warmup and steps.
+    s = rhydro.Solver(gamma=gamma, slope_type=1, riemann="hllc", precision=rhydro.RH_PREC_MIXED)
+    t_loop = 0.0
+    same = True
+    for b, (m, u) in enumerate(zip(meshes, u0)):
+        m.uold[:] = u
+        m.unew[:] = u
+        s.bind(m, rhydro.RH_DEVICE_RESIDENT)
+        s.run_uniform(levels[b], dxs[b], cf, warm)
+        t0 = time.perf_counter()
+        s.run_uniform(levels[b], dxs[b], cf, args.steps)
+        t_loop += time.perf_counter() - t0
+        s.sync_to_host()
+        same = same and bool(np.array_equal(m.uold, finals["mixed"][b]))
+    s.close()
+    unbatched = nleaf * args.steps / t_loop
+    print(f"[bench] unbatched mixed loop: {t_loop:.4f}s, identical={same}", file=sys.stderr, flush=True)
+    return {
+        "workload": f"shock-tube ensemble: {args.boxes} periodic cubic boxes, sides {sides[0]}-{sides[-1]} cells "
+                    f"(log-uniform, nx*2^(L-1) with nx in 1,3,5,7; seed {args.seed}), {nleaf} leaf cells, "
+                    "random plane normal, rho and p in [0.1, 1] per side, gamma=1.4, HLLC, slope_type 1, "
+                    "own Courant time step per box",
+        "cells": int(nleaf),
+        "value_mixed": out["mixed"]["value"],
+        "value_fp64": out["fp64"]["value"],
+        "mixed": out["mixed"],
+        "fp64": out["fp64"],
+        "unbatched_mixed_value": unbatched,
+        "batched_speedup": out["mixed"]["value"] / unbatched,
+        "batched_equals_unbatched": same,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +138,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip the mixed-precision form")
+    ap.add_argument("--boxes", type=int, default=96, help="boxes of the shock-tube ensemble (0: skip it)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -163,6 +241,8 @@ def main() -> int:
                           f"{args.warmup}+{args.steps} steps with each form's own time steps",
         }
 
+    ens_info = measure_ensemble(args, rhydro, np) if args.boxes > 0 else None
+
     value = nleaf * args.steps / fp64["wall"]
     achieved = fp64["achieved"]
     g_ms = fp64["kernel_ms"]["godunov"]
@@ -222,6 +302,7 @@ def main() -> int:
             "kernel_ms": fp64["kernel_ms"],
             "autotune": fp64["autotune"],
             "mixed": mixed_info,
+            "ensemble": ens_info,
         },
         "roofline": {
             "bound": "hbm",

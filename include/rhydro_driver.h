@@ -47,6 +47,14 @@ int rh_init_regions(const rh_mesh* mesh, const double* xg, double boxlen, double
                     const double* u_region, const double* v_region, const double* w_region,
                     const double* p_region);
 
+/* Shock-tube initial condition with an arbitrary plane: cells whose centre x
+ * has (x - point).normal < 0 get the primitive state left[5] = (d, u, v, w,
+ * p), the others right[5]; cell centres, the upward restriction and
+ * unew = uold as in rh_init_regions.  For the ensemble benchmark's Sod boxes
+ * (no counterpart in the reference's region types). */
+int rh_init_shock_tube(const rh_mesh* mesh, const double* xg, double boxlen, double gamma, const double* normal,
+                       const double* point, const double* left, const double* right);
+
 /* Seeded random state on the leaf cells of level list `level`, in list order
  * and cell order: rho = exp(s*Z1), p = exp(s*Z2), v_i = a*c_s*Z with
  * c_s = sqrt(gamma*p/rho), E = p/(gamma-1) + rho*v^2/2, where the Z are

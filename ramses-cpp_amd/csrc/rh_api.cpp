@@ -10,61 +10,20 @@
 #include <string>
 #include <vector>
 
+#include "rh_context.h"
 #include "rh_kernels.h"
 #include "rh_tables.h"
 #include "rhydro.h"
 #include "rhydro_driver.h"
+#include "rhydro_ensemble.h"
 
 static_assert(rh::REC_REFINED == rh::REC_REFINED_DEV, "record flags");
 static_assert(rh::REC_BOUNDARY == rh::REC_BOUNDARY_DEV, "record flags");
 static_assert(rh::REC_REFLUX == rh::REC_REFLUX_DEV, "record flags");
 
-namespace {
+namespace rhi {
 
 std::string g_create_error;
-
-struct DevLevel {
-    long long epoch = -2;
-    rh::LevelDev dev;
-    std::vector<void*> allocs;
-};
-
-enum { TK_GODUNOV = 0, TK_COURANT = 1, TK_SET_UNEW = 2, TK_SET_UOLD = 3, TK_N = 4 };
-
-}  // namespace
-
-struct rh_ctx {
-    rh_params p{};
-    int mode = RH_HOST_COHERENT;
-    hipStream_t stream = nullptr;
-    std::string err;
-    bool bound = false;
-    rh_mesh mesh{};
-    double* d_uold = nullptr;
-    double* d_unew = nullptr;
-    double* d_f = nullptr;
-    size_t state_cap = 0;  // doubles per state array
-    size_t f_cap = 0;
-    rh::TopoIndex topo;
-    bool topo_valid = false;
-    std::vector<DevLevel> levels;
-    double* d_part = nullptr;
-    int npart = 0;
-    double* d_dt = nullptr;
-    int ndt = 0;
-    double* d_u1 = nullptr;
-    double* d_u2 = nullptr;
-    int interpol_cap = 0;
-    int variant = 1;  // Godunov kernel variant (rh_set_kernel_variant)
-    bool timers = false;
-    double t_ms[TK_N] = {0, 0, 0, 0};
-    long long t_n[TK_N] = {0, 0, 0, 0};
-    // Event pairs recorded around launches, summed when the timers are read.
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
-};
-
-namespace {
 
 int fail(rh_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -76,11 +35,6 @@ int hip_fail(rh_ctx* c, hipError_t e, const char* what) {
     return fail(c, RH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define RH_HIP(c, call)                                    \
-    do {                                                   \
-        hipError_t _e = (call);                            \
-        if (_e != hipSuccess) return hip_fail(c, _e, #call); \
-    } while (0)
 
 void free_level(DevLevel& L) {
     for (void* p : L.allocs) (void)hipFree(p);
@@ -89,17 +43,6 @@ void free_level(DevLevel& L) {
     L.epoch = -2;
 }
 
-template <typename T> hipError_t upload(DevLevel& L, T*& dst, const std::vector<T>& src, hipStream_t s) {
-    dst = nullptr;
-    if (src.empty()) return hipSuccess;
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, src.size() * sizeof(T));
-    if (e != hipSuccess) return e;
-    L.allocs.push_back(p);
-    dst = (T*)p;
-    e = hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s);
-    return e;
-}
 
 rh::MeshDev mesh_dev(const rh_ctx* c) {
     rh::MeshDev m;
@@ -194,6 +137,7 @@ int ensure_dt(rh_ctx* c, int n) {
 size_t state_bytes(const rh_ctx* c) { return (size_t)c->mesh.nvar * c->mesh.ncell * sizeof(double); }
 
 int upload_state(rh_ctx* c, int what) {
+    if (c->ens) return ens_upload_state(c, what);
     if (what & RH_UOLD)
         RH_HIP(c, hipMemcpyAsync(c->d_uold, c->mesh.uold, state_bytes(c), hipMemcpyHostToDevice, c->stream));
     if (what & RH_UNEW)
@@ -202,6 +146,7 @@ int upload_state(rh_ctx* c, int what) {
 }
 
 int download_state(rh_ctx* c, int what) {
+    if (c->ens) return ens_download_state(c, what);
     if (what & RH_UOLD)
         RH_HIP(c, hipMemcpyAsync(c->mesh.uold, c->d_uold, state_bytes(c), hipMemcpyDeviceToHost, c->stream));
     if (what & RH_UNEW)
@@ -213,6 +158,7 @@ int download_state(rh_ctx* c, int what) {
 int check_bound(rh_ctx* c) {
     if (!c) return RH_ERR_ARG;
     if (!c->bound) return fail(c, RH_ERR_ARG, "no mesh bound (call rh_bind first)");
+    if (c->ens) return fail(c, RH_ERR_ARG, "an ensemble is bound: its boxes advance through rh_run_ensemble");
     return RH_OK;
 }
 
@@ -255,23 +201,30 @@ void collect_timers(rh_ctx* c) {
     c->ev_pending.clear();
 }
 
-struct Timed {
-    rh_ctx* c;
-    int k;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    Timed(rh_ctx* cc, int kk) : c(cc), k(kk) {
-        if (!c->timers) return;
-        e0 = take_event(c);
-        e1 = take_event(c);
-        if (e0) (void)hipEventRecord(e0, c->stream);
-    }
-    void done() {
-        if (!c->timers || !e0 || !e1) return;
-        (void)hipEventRecord(e1, c->stream);
-        c->ev_pending.push_back({k, {e0, e1}});
-        if (c->ev_pending.size() > 4096) collect_timers(c);
-    }
-};
+double exact_inverse(double dx) {
+    int e = 0;
+    const double m = std::frexp(dx, &e);
+    if (!(dx > 0.0) || m != 0.5 || e < -1000 || e > 1000) return 0.0;
+    return 1.0 / dx;
+}
+
+void unbind(rh_ctx* c) {
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& L : c->levels) free_level(L);
+    free_level(c->ens_level);
+    for (void* p : {(void*)c->d_uold, (void*)c->d_unew, (void*)c->d_f, (void*)c->d_boxlen, (void*)c->d_keys})
+        if (p) (void)hipFree(p);
+    c->d_uold = c->d_unew = c->d_f = c->d_boxlen = nullptr;
+    c->d_keys = nullptr;
+    c->state_cap = c->f_cap = 0;
+    c->nkeys = 0;
+    c->topo_valid = false;
+    c->ens = false;
+    c->boxes.clear();
+    c->ens_levels.clear();
+    c->bound = false;
+    c->mesh = rh_mesh{};
+}
 
 int do_courant(rh_ctx* c, const rh::LevelDev* lv, double dx, double gamma, double cf, double* d_out) {
     int st = ensure_part(c, rh::courant_blocks(*lv) + 1);
@@ -286,14 +239,16 @@ int do_courant(rh_ctx* c, const rh::LevelDev* lv, double dx, double gamma, doubl
 int do_godunov(rh_ctx* c, const rh::LevelDev* lv, const double* d_dt, double dx) {
     Timed tm(c, TK_GODUNOV);
     hipError_t e = rh::launch_godunov(mesh_dev(c), *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision,
-                                      c->variant, d_dt, dx, c->stream);
+                                      c->variant, d_dt, dx, exact_inverse(dx), c->stream);
     if (e == hipErrorNotSupported) return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
     if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
     tm.done();
     return RH_OK;
 }
 
-}  // namespace
+}  // namespace rhi
+
+using namespace rhi;
 
 extern "C" {
 
@@ -328,9 +283,8 @@ int rh_destroy(rh_ctx* c) {
     if (!c) return RH_OK;
     (void)hipSetDevice(c->p.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto& L : c->levels) free_level(L);
-    for (void* p : {(void*)c->d_uold, (void*)c->d_unew, (void*)c->d_f, (void*)c->d_part, (void*)c->d_dt,
-                    (void*)c->d_u1, (void*)c->d_u2})
+    unbind(c);
+    for (void* p : {(void*)c->d_part, (void*)c->d_dt, (void*)c->d_u1, (void*)c->d_u2})
         if (p) (void)hipFree(p);
     collect_timers(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -350,6 +304,7 @@ int rh_bind(rh_ctx* c, const rh_mesh* m, int mode) {
     if (!m->father || !m->son || !m->nbor || !m->next || !m->headl || !m->uold || !m->unew)
         return fail(c, RH_ERR_ARG, "null mesh array");
     RH_HIP(c, hipSetDevice(c->p.device));
+    if (c->ens) unbind(c);
     const size_t n = (size_t)m->nvar * m->ncell;
     bool topo_changed = !c->bound || c->mesh.mesh_epoch != m->mesh_epoch || c->mesh.father != m->father ||
                         c->mesh.son != m->son || c->mesh.nbor != m->nbor || c->mesh.next != m->next ||
@@ -472,8 +427,9 @@ int rh_interpol_hydro(rh_ctx* c, int n_octs, const double* u1, double* u2) {
 }
 
 int rh_sync(rh_ctx* c, int direction, int what) {
-    int st = check_bound(c);
-    if (st) return st;
+    if (!c) return RH_ERR_ARG;
+    if (!c->bound) return fail(c, RH_ERR_ARG, "no mesh bound (call rh_bind first)");
+    int st;
     if (direction == RH_SYNC_TO_HOST) return download_state(c, what);
     if (direction == RH_SYNC_TO_DEVICE) {
         if ((st = upload_state(c, what))) return st;

@@ -1,0 +1,136 @@
+// rh_context.h -- the context behind the C-ABI handle (rh_ctx) and the host
+// helpers shared by the single-mesh entry points (rh_api.cpp) and the
+// ensemble form (rh_ensemble.cpp).  Internal to the library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "rh_kernels.h"
+#include "rh_tables.h"
+#include "rhydro.h"
+
+namespace rhi {
+
+struct DevLevel {
+    long long epoch = -2;
+    rh::LevelDev dev;
+    std::vector<void*> allocs;
+};
+
+enum { TK_GODUNOV = 0, TK_COURANT = 1, TK_SET_UNEW = 2, TK_SET_UOLD = 3, TK_N = 4 };
+
+// One mesh of an ensemble: its host arrays and where its cells sit in the
+// packed device state (coarse cells from coff+1, grids from goff+1).
+struct EnsBox {
+    rh_mesh mesh{};
+    long long coff = 0, goff = 0;
+};
+
+}  // namespace rhi
+
+struct rh_ctx {
+    rh_params p{};
+    int mode = RH_HOST_COHERENT;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool bound = false;
+    // Single mesh: the caller's arrays.  Ensemble: a packed forest of every
+    // box (ncoarse, ngridmax, ncell summed; no host arrays).
+    rh_mesh mesh{};
+    double* d_uold = nullptr;
+    double* d_unew = nullptr;
+    double* d_f = nullptr;
+    size_t state_cap = 0;  // doubles per state array
+    size_t f_cap = 0;
+    rh::TopoIndex topo;
+    bool topo_valid = false;
+    std::vector<rhi::DevLevel> levels;
+    double* d_part = nullptr;
+    int npart = 0;
+    double* d_dt = nullptr;
+    int ndt = 0;
+    double* d_u1 = nullptr;
+    double* d_u2 = nullptr;
+    int interpol_cap = 0;
+    int variant = 1;  // Godunov kernel variant (rh_set_kernel_variant)
+    bool timers = false;
+    double t_ms[rhi::TK_N] = {0, 0, 0, 0};
+    long long t_n[rhi::TK_N] = {0, 0, 0, 0};
+    // Event pairs recorded around launches, summed when the timers are read.
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    // Ensemble form (rh_bind_ensemble / rh_run_ensemble).
+    bool ens = false;
+    std::vector<rhi::EnsBox> boxes;
+    std::vector<int> ens_levels;  // level of each box the tables were built for
+    rhi::DevLevel ens_level;      // concatenated tables of every box
+    double* d_boxlen = nullptr;   // [2*nbox]: dx, then 1/dx (0 when not exact)
+    unsigned long long* d_keys = nullptr;  // [nkeys]: ordered keys of the per-box dt
+    int nkeys = 0;
+};
+
+namespace rhi {
+
+extern std::string g_create_error;
+
+int fail(rh_ctx* c, int code, const std::string& msg);
+int hip_fail(rh_ctx* c, hipError_t e, const char* what);
+
+#define RH_HIP(c, call)                                      \
+    do {                                                     \
+        hipError_t _e = (call);                              \
+        if (_e != hipSuccess) return rhi::hip_fail(c, _e, #call); \
+    } while (0)
+
+void free_level(DevLevel& L);
+rh::MeshDev mesh_dev(const rh_ctx* c);
+rh::GasParams gas(const rh_ctx* c, double gamma);
+int ensure_dt(rh_ctx* c, int n);
+void collect_timers(rh_ctx* c);
+// Frees every device buffer of the current binding (single or ensemble).
+void unbind(rh_ctx* c);
+
+// Allocates a device copy of src, owned by L (freed by free_level).
+template <typename T> hipError_t upload(DevLevel& L, T*& dst, const std::vector<T>& src, hipStream_t s) {
+    dst = nullptr;
+    if (src.empty()) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, src.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    L.allocs.push_back(p);
+    dst = (T*)p;
+    return hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+// When dx is a normal power of two, its reciprocal (exact) for the kernels'
+// multiply-instead-of-divide path; 0 otherwise.
+double exact_inverse(double dx);
+
+// HIP events around one launch on the context's stream, when timers are on.
+hipEvent_t take_event(rh_ctx* c);
+struct Timed {
+    rh_ctx* c;
+    int k;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Timed(rh_ctx* cc, int kk) : c(cc), k(kk) {
+        if (!c->timers) return;
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (e0) (void)hipEventRecord(e0, c->stream);
+    }
+    void done() {
+        if (!c->timers || !e0 || !e1) return;
+        (void)hipEventRecord(e1, c->stream);
+        c->ev_pending.push_back({k, {e0, e1}});
+        if (c->ev_pending.size() > 4096) collect_timers(c);
+    }
+};
+
+// Ensemble state transfer (rh_ensemble.cpp).
+int ens_upload_state(rh_ctx* c, int what);
+int ens_download_state(rh_ctx* c, int what);
+
+}  // namespace rhi

@@ -30,6 +30,67 @@ int headl_of(const rh_mesh& m, int il) {
     return m.headl[(size_t)il * (m.ncpu + m.nboundary) + (m.myid - 1)];
 }
 
+
+// Calls fn(icell, x, y, z) for every coarse cell and every cell of every
+// level list, at the cell centres AmrGrid::get_cell_center gives
+// (AmrGrid.cpp:333-363); then restricts upward as restrict_fine does for
+// il = nlevelmax-1 .. 0 (TreeUpdater.cpp:195-215) and sets unew = uold.
+template <typename Fn> void fill_cells(const rh_mesh& m, const double* xg, double boxlen, Fn fn) {
+    auto U = [&](int icell, int iv) -> double& { return m.uold[(size_t)(iv - 1) * m.ncell + icell - 1]; };
+    int nmax = m.nx > m.ny ? m.nx : m.ny;
+    nmax = nmax > m.nz ? nmax : m.nz;
+    double dx_coarse = boxlen / nmax;
+    for (int i = 1; i <= m.ncoarse; ++i) {
+        int idx = i - 1;
+        int iz = idx / (m.nx * m.ny);
+        idx %= (m.nx * m.ny);
+        int iy = idx / m.nx, ix = idx % m.nx;
+        fn(i, (ix + 0.5) * dx_coarse, (iy + 0.5) * dx_coarse, (iz + 0.5) * dx_coarse);
+    }
+    for (int il = 1; il <= m.nlevelmax; ++il) {
+        for (int ig = headl_of(m, il); ig > 0; ig = m.next[ig - 1]) {
+            int level = grid_depth(m, ig);
+            double dx = boxlen / (double)(m.nx * (1 << level));
+            double scale = boxlen / (double)m.nx;
+            for (int ic0 = 0; ic0 < 8; ++ic0) {
+                int ixyz[3] = {ic0 & 1, (ic0 & 2) >> 1, (ic0 & 4) >> 2};
+                double xc[3];
+                for (int d = 0; d < 3; ++d)
+                    xc[d] = xg[(size_t)d * m.ngridmax + ig - 1] * scale + (double)(ixyz[d] - 0.5) * dx;
+                fn(cell_of(m.ncoarse, m.ngridmax, ig, ic0), xc[0], xc[1], xc[2]);
+            }
+        }
+    }
+    for (int il = m.nlevelmax - 1; il >= 0; --il) {
+        for (int ig = headl_of(m, il + 1); ig > 0; ig = m.next[ig - 1]) {
+            int fc = m.father[ig - 1];
+            if (fc <= 0) continue;
+            for (int iv = 1; iv <= m.nvar; ++iv) {
+                double sum = 0;
+                for (int ic0 = 0; ic0 < 8; ++ic0) sum += U(cell_of(m.ncoarse, m.ngridmax, ig, ic0), iv);
+                U(fc, iv) = sum / 8;
+            }
+        }
+    }
+    for (size_t i = 0; i < (size_t)m.nvar * m.ncell; ++i) m.unew[i] = m.uold[i];
+}
+
+// Conservative state of (d, u, v, w, p) into cell idc (Initializer.cpp:104-121).
+void set_prim(const rh_mesh& m, int idc, double gamma, double dr, double ur, double vr, double wr, double pr) {
+    auto U = [&](int iv) -> double& { return m.uold[(size_t)(iv - 1) * m.ncell + idc - 1]; };
+    U(1) = dr;
+    U(2) = dr * ur;
+    U(3) = dr * vr;
+    U(4) = dr * wr;
+    double v2 = ur * ur;
+    v2 += vr * vr;
+    v2 += wr * wr;
+    double e_kin = 0.5 * dr * v2;
+    double e_int = pr / (gamma - 1.0);
+    double e_nonthermal = 0.0;
+    U(5) = e_kin + e_int + e_nonthermal;
+}
+
 }  // namespace
 
 extern "C" {
@@ -139,8 +200,7 @@ int rh_init_regions(const rh_mesh* mp, const double* xg, double boxlen, double g
                     const double* p_region) {
     if (!mp || !xg || nreg < 0) return RH_ERR_ARG;
     const rh_mesh& m = *mp;
-    auto U = [&](int icell, int iv) -> double& { return m.uold[(size_t)(iv - 1) * m.ncell + icell - 1]; };
-    auto apply = [&](int idc, double x, double y, double z) {
+    fill_cells(m, xg, boxlen, [&](int idc, double x, double y, double z) {
         for (int ir = 0; ir < nreg; ++ir) {
             bool in = false;
             if (type[ir] == 0) {
@@ -153,59 +213,21 @@ int rh_init_regions(const rh_mesh* mp, const double* xg, double boxlen, double g
                 in = r2 < std::pow(length_x[ir], 2);
             }
             if (!in) continue;
-            double dr = d_region[ir], ur = u_region[ir], vr = v_region[ir], wr = w_region[ir];
-            U(idc, 1) = dr;
-            U(idc, 2) = dr * ur;
-            U(idc, 3) = dr * vr;
-            U(idc, 4) = dr * wr;
-            double v2 = ur * ur;
-            v2 += vr * vr;
-            v2 += wr * wr;
-            double e_kin = 0.5 * dr * v2;
-            double e_int = p_region[ir] / (gamma - 1.0);
-            double e_nonthermal = 0.0;
-            U(idc, 5) = e_kin + e_int + e_nonthermal;
+            set_prim(m, idc, gamma, d_region[ir], u_region[ir], v_region[ir], w_region[ir], p_region[ir]);
         }
-    };
-    // Coarse cells (AmrGrid.cpp:333-340).
-    int nmax = m.nx > m.ny ? m.nx : m.ny;
-    nmax = nmax > m.nz ? nmax : m.nz;
-    double dx_coarse = boxlen / nmax;
-    for (int i = 1; i <= m.ncoarse; ++i) {
-        int idx = i - 1;
-        int iz = idx / (m.nx * m.ny);
-        idx %= (m.nx * m.ny);
-        int iy = idx / m.nx, ix = idx % m.nx;
-        apply(i, (ix + 0.5) * dx_coarse, (iy + 0.5) * dx_coarse, (iz + 0.5) * dx_coarse);
-    }
-    // Oct cells of every level list (AmrGrid.cpp:341-363).
-    for (int il = 1; il <= m.nlevelmax; ++il) {
-        for (int ig = headl_of(m, il); ig > 0; ig = m.next[ig - 1]) {
-            int level = grid_depth(m, ig);
-            double dx = boxlen / (double)(m.nx * (1 << level));
-            double scale = boxlen / (double)m.nx;
-            for (int ic0 = 0; ic0 < 8; ++ic0) {
-                int ixyz[3] = {ic0 & 1, (ic0 & 2) >> 1, (ic0 & 4) >> 2};
-                double xc[3];
-                for (int d = 0; d < 3; ++d)
-                    xc[d] = xg[(size_t)d * m.ngridmax + ig - 1] * scale + (double)(ixyz[d] - 0.5) * dx;
-                apply(cell_of(m.ncoarse, m.ngridmax, ig, ic0), xc[0], xc[1], xc[2]);
-            }
-        }
-    }
-    // restrict_fine(il) for il = nlevelmax-1 .. 0 (TreeUpdater.cpp:195-215).
-    for (int il = m.nlevelmax - 1; il >= 0; --il) {
-        for (int ig = headl_of(m, il + 1); ig > 0; ig = m.next[ig - 1]) {
-            int fc = m.father[ig - 1];
-            if (fc <= 0) continue;
-            for (int iv = 1; iv <= m.nvar; ++iv) {
-                double sum = 0;
-                for (int ic0 = 0; ic0 < 8; ++ic0) sum += U(cell_of(m.ncoarse, m.ngridmax, ig, ic0), iv);
-                U(fc, iv) = sum / 8;
-            }
-        }
-    }
-    for (size_t i = 0; i < (size_t)m.nvar * m.ncell; ++i) m.unew[i] = m.uold[i];
+    });
+    return RH_OK;
+}
+
+int rh_init_shock_tube(const rh_mesh* mp, const double* xg, double boxlen, double gamma, const double* normal,
+                       const double* point, const double* left, const double* right) {
+    if (!mp || !xg || !normal || !point || !left || !right) return RH_ERR_ARG;
+    const rh_mesh& m = *mp;
+    fill_cells(m, xg, boxlen, [&](int idc, double x, double y, double z) {
+        const double s = (x - point[0]) * normal[0] + (y - point[1]) * normal[1] + (z - point[2]) * normal[2];
+        const double* q = s < 0.0 ? left : right;
+        set_prim(m, idc, gamma, q[0], q[1], q[2], q[3], q[4]);
+    });
     return RH_OK;
 }
 

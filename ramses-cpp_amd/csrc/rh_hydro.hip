@@ -80,7 +80,32 @@ RH_DEV double wave_min(double v) {
     return v;
 }
 
-// HydroSolver.cpp:878-977 per leaf cell, then a block minimum.
+// Courant candidate of one leaf cell, HydroSolver.cpp:878-977.
+RH_DEV double courant_cand(const MeshDev& m, int id, double dx, double gamma, double courant_factor) {
+    double u[5];
+    load_u(m.uold, m.ncell, id, u);
+    double d = smax(u[0], 1e-10), v2 = 0.0;
+    for (int i = 1; i <= 3; ++i) {
+        double v = u[i] / d;
+        v2 = v2 + v * v;
+    }
+    double sum_gamma_p = 0.0;
+    double e_nonthermal = 0.0;
+    double pr = smax((u[4] - 0.5 * d * v2 - e_nonthermal) * (gamma - 1.0), d * 1e-10);
+    pr = smin(pr, d * 1e2);
+    double c2 = smax(gamma * pr / smax(d, 1e-10), 1e-20);
+    c2 = c2 + sum_gamma_p / d;
+    double cs = ssqrt(smax(c2, 1e-10));
+    double u_wave = 3 * cs;
+    for (int i = 1; i <= 3; ++i) u_wave = u_wave + sabs(u[i] / d);
+    double a_grav = 0.0;
+    for (int i = 1; i <= 3; ++i) a_grav = a_grav + sabs(m.f ? m.f[(size_t)(i - 1) * m.ncell + id - 1] : 0.0);
+    double alpha = a_grav * dx / (u_wave * u_wave);
+    alpha = smax(alpha, 0.0001);
+    return (dx / u_wave) * (ssqrt(1.0 + 2.0 * courant_factor * alpha) - 1.0) / alpha;
+}
+
+// Per leaf cell, then a block minimum.
 __global__ __launch_bounds__(BLOCK) void k_courant_partial(MeshDev m, LevelDev lv, double dx, double gamma,
                                                            double courant_factor, double* part) {
     __shared__ double s_min[BLOCK / 64];
@@ -88,31 +113,8 @@ __global__ __launch_bounds__(BLOCK) void k_courant_partial(MeshDev m, LevelDev l
     const int p = blockIdx.x * OCTS_PER_BLOCK + (t % OCTS_PER_BLOCK);
     const int ic0 = t / OCTS_PER_BLOCK;
     double cand = __builtin_inf();
-    if (p < lv.noct && ((lv.leaf[p] >> ic0) & 1)) {
-        const int id = cell_id(m, lv.grid[p], ic0);
-        double u[5];
-        load_u(m.uold, m.ncell, id, u);
-        double d = smax(u[0], 1e-10), v2 = 0.0;
-        for (int i = 1; i <= 3; ++i) {
-            double v = u[i] / d;
-            v2 = v2 + v * v;
-        }
-        double sum_gamma_p = 0.0;
-        double e_nonthermal = 0.0;
-        double pr = smax((u[4] - 0.5 * d * v2 - e_nonthermal) * (gamma - 1.0), d * 1e-10);
-        pr = smin(pr, d * 1e2);
-        double c2 = smax(gamma * pr / smax(d, 1e-10), 1e-20);
-        c2 = c2 + sum_gamma_p / d;
-        double cs = ssqrt(smax(c2, 1e-10));
-        double u_wave = 3 * cs;
-        for (int i = 1; i <= 3; ++i) u_wave = u_wave + sabs(u[i] / d);
-        double a_grav = 0.0;
-        for (int i = 1; i <= 3; ++i) a_grav = a_grav + sabs(m.f ? m.f[(size_t)(i - 1) * m.ncell + id - 1] : 0.0);
-        double alpha = a_grav * dx / (u_wave * u_wave);
-        alpha = smax(alpha, 0.0001);
-        double dt_cand = (dx / u_wave) * (ssqrt(1.0 + 2.0 * courant_factor * alpha) - 1.0) / alpha;
-        cand = min_skip_nan(cand, dt_cand);
-    }
+    if (p < lv.noct && ((lv.leaf[p] >> ic0) & 1))
+        cand = min_skip_nan(cand, courant_cand(m, cell_id(m, lv.grid[p], ic0), dx, gamma, courant_factor));
     cand = wave_min(cand);
     if ((t & 63) == 0) s_min[t >> 6] = cand;
     __syncthreads();
@@ -121,6 +123,65 @@ __global__ __launch_bounds__(BLOCK) void k_courant_partial(MeshDev m, LevelDev l
         for (int w = 1; w < BLOCK / 64; ++w) v = min_skip_nan(v, s_min[w]);
         part[blockIdx.x] = v;
     }
+}
+
+// Ordered 64-bit key of a double: unsigned comparison of keys orders them as
+// the doubles (-0 before +0), and NaN maps to the largest key, so a minimum
+// over keys never picks a NaN candidate -- the reference's std::min loop
+// skips them too.  The minimum of a set is order-independent, so atomics
+// give a deterministic result.
+RH_DEV unsigned long long dt_key(double v) {
+    if (v != v) return ~0ull;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+RH_DEV double key_dt(unsigned long long k) {
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)b);
+}
+
+// Ensemble Courant step: per-box minimum into keys[box].  Octs of a box are
+// contiguous in the concatenated list, so a block whose first and last oct
+// share a box reduces in registers and LDS and issues one atomic; a block
+// straddling boxes lets every leaf cell issue its own.
+__global__ __launch_bounds__(BLOCK) void k_courant_boxes(MeshDev m, LevelDev lv, double gamma, double courant_factor,
+                                                         unsigned long long* keys) {
+    __shared__ double s_min[BLOCK / 64];
+    const int t = threadIdx.x;
+    const int p0 = blockIdx.x * OCTS_PER_BLOCK;
+    const int p = p0 + (t % OCTS_PER_BLOCK);
+    const int ic0 = t / OCTS_PER_BLOCK;
+    const int plast = min(p0 + OCTS_PER_BLOCK, lv.noct) - 1;
+    const int b0 = lv.obox[p0], b1 = lv.obox[plast];
+    double cand = __builtin_inf();
+    int bx = b0;
+    if (p < lv.noct && ((lv.leaf[p] >> ic0) & 1)) {
+        bx = lv.obox[p];
+        cand = min_skip_nan(cand, courant_cand(m, cell_id(m, lv.grid[p], ic0), lv.box_len[bx], gamma,
+                                               courant_factor));
+    }
+    if (b0 == b1) {
+        cand = wave_min(cand);
+        if ((t & 63) == 0) s_min[t >> 6] = cand;
+        __syncthreads();
+        if (t == 0) {
+            double v = s_min[0];
+            for (int w = 1; w < BLOCK / 64; ++w) v = min_skip_nan(v, s_min[w]);
+            atomicMin(&keys[b0], dt_key(v));
+        }
+    } else if (cand != __builtin_inf()) {
+        atomicMin(&keys[bx], dt_key(cand));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_keys(unsigned long long* keys, int n, double value) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) keys[i] = dt_key(value);
+}
+
+__global__ __launch_bounds__(256) void k_keys_to_dt(const unsigned long long* keys, int n, double* dt) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dt[i] = key_dt(keys[i]);
 }
 
 __global__ __launch_bounds__(256) void k_courant_final(const double* part, int npart, double dt_init, double* dt) {
@@ -140,6 +201,19 @@ __global__ __launch_bounds__(256) void k_courant_final(const double* part, int n
 
 // ---------------------------------------------------------------------------
 // Godunov update (FP64, bit-faithful).
+
+// Time step and cell width of oct p: the kernel arguments for a single mesh,
+// its box's entries for an ensemble.
+RH_DEV void step_of(const LevelDev& lv, int p, const double* d_dt, double& dt, double& dx, double& inv_dx) {
+    if (lv.obox) {
+        const int bx = lv.obox[p];
+        dt = d_dt[bx];
+        dx = lv.box_len[bx];
+        inv_dx = lv.box_len[lv.nbox + bx];
+    } else {
+        dt = *d_dt;
+    }
+}
 
 struct Nbr {
     int cell;  // 1-based cell index (<= 0: boundary)
@@ -259,9 +333,9 @@ RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const dou
 // and the two neighbour octs, computes its own two face fluxes and the two
 // fluxes its same-level neighbours L1/R1 compute for the shared faces, then
 // adds everything into unew in the reference's processing order.
-template <int RIEMANN, bool MIXED>
+template <int RIEMANN, bool MIXED, bool EXACT>
 __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
-                                                      double dx) {
+                                                      double dx, double inv_dx) {
     using TC = typename std::conditional<MIXED, float, double>::type;
     // Per-thread reflux payload: the six incoming contributions and their
     // processing keys, kept in LDS so they can be read back in sorted order.
@@ -276,9 +350,11 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
     const Gas<double> gd{gp.gamma, gp.gm1, gp.smalle, gp.smallp, 1.0 / gp.gm1};
     const Gas<TC> gc{(TC)gp.gamma, (TC)gp.gm1, (TC)gp.smalle, (TC)gp.smallp, (TC)(1.0 / gp.gm1)};
     const int slope_type = gp.slope_type, interpol_type = gp.interpol_type;
-    const double dt = *d_dt;
+    double dt;
+    step_of(lv, p, d_dt, dt, dx, inv_dx);
     const double dtdx = dt / dx;
     const TC dtc = (TC)dt, dxc = (TC)dx;
+    const Len<TC, EXACT> len{dxc, (TC)inv_dx};
     const int noct = lv.noct;
     const int cC = cell_id(m, lv.grid[p], ic0);
 
@@ -355,8 +431,8 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
         // Own face states (trace pass) and this cell's ghost states as its
         // neighbours see them (coarse-ghost branch with n = this cell).
         TC OCm[5], OCp[5], GCm[5], GCp[5];
-        own_trace(qL1, qC, qR1, d, slope_type, dtc, dxc, OCm, OCp, gc);
-        ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, dxc, GCm, GCp, gc);
+        own_trace(qL1, qC, qR1, d, slope_type, dtc, len, OCm, OCp, gc);
+        ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, len, GCm, GCp, gc);
 
         // ---- minus face (side 0): ql = ghost qm of L1, qr = own qp
         {
@@ -368,13 +444,13 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
                     const int L2cell = A >= 0 ? cell_id(m, lv.grid[A], ic0) : lv.rec[-1 - A].x;
                     prim(L2cell, qL1, qL2);
                     TC Gm[5], Gp[5];
-                    ghost_trace(qL2, qL1, qC, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+                    ghost_trace(qL2, qL1, qC, d, interpol_type, dtc, len, Gm, Gp, gc);
                     solve_face<RIEMANN, MIXED>(Gm, OCp, F, gc, gd);
                     rotate(F, d);
                     // L1's own plus-face flux into this cell (reflux).
                     TC Om[5], Op[5];
                     double Fin[5];
-                    own_trace(qL2, qL1, qC, d, slope_type, dtc, dxc, Om, Op, gc);
+                    own_trace(qL2, qL1, qC, d, slope_type, dtc, len, Om, Op, gc);
                     solve_face<RIEMANN, MIXED>(Om, GCp, Fin, gc, gd);
                     rotate(Fin, d);
 #pragma unroll
@@ -395,7 +471,7 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
                     prim(rc.y, qn, qnl);
                     prim(rc.z, qn, qnr);
                     TC Gm[5], Gp[5];
-                    ghost_trace(qnl, qn, qnr, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+                    ghost_trace(qnl, qn, qnr, d, interpol_type, dtc, len, Gm, Gp, gc);
                     solve_face<RIEMANN, MIXED>(Gm, OCp, F, gc, gd);
                     rotate(F, d);
                     const int slot = lv.rec_slot[L1.rec] + face_slot(ic0, d);
@@ -417,13 +493,13 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
                     const int R2cell = B >= 0 ? cell_id(m, lv.grid[B], ic0) : lv.rec[-1 - B].x;
                     prim(R2cell, qR1, qR2);
                     TC Gm[5], Gp[5];
-                    ghost_trace(qC, qR1, qR2, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+                    ghost_trace(qC, qR1, qR2, d, interpol_type, dtc, len, Gm, Gp, gc);
                     solve_face<RIEMANN, MIXED>(OCm, Gp, F, gc, gd);
                     rotate(F, d);
                     // R1's own minus-face flux into this cell (reflux).
                     TC Om[5], Op[5];
                     double Fin[5];
-                    own_trace(qC, qR1, qR2, d, slope_type, dtc, dxc, Om, Op, gc);
+                    own_trace(qC, qR1, qR2, d, slope_type, dtc, len, Om, Op, gc);
                     solve_face<RIEMANN, MIXED>(GCm, Op, Fin, gc, gd);
                     rotate(Fin, d);
 #pragma unroll
@@ -444,7 +520,7 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
                     prim(rc.y, qn, qnl);
                     prim(rc.z, qn, qnr);
                     TC Gm[5], Gp[5];
-                    ghost_trace(qnl, qn, qnr, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+                    ghost_trace(qnl, qn, qnr, d, interpol_type, dtc, len, Gm, Gp, gc);
                     solve_face<RIEMANN, MIXED>(OCm, Gp, F, gc, gd);
                     rotate(F, d);
                     const int slot = lv.rec_slot[R1.rec] + face_slot(ic0, d);
@@ -482,9 +558,9 @@ template <typename T> RH_DEV void sel5(bool c, const T a[5], const T b[5], T out
 constexpr int O_OCTS = 16;
 constexpr int O_BLOCK = O_OCTS * 8;
 
-template <int RIEMANN, bool MIXED>
+template <int RIEMANN, bool MIXED, bool EXACT>
 __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
-                                                          double dx) {
+                                                          double dx, double inv_dx) {
     using TC = typename std::conditional<MIXED, float, double>::type;
     __shared__ double s_r[30][O_BLOCK];
     __shared__ long long s_k[6][O_BLOCK];
@@ -502,9 +578,11 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
     const Gas<double> gd{gp.gamma, gp.gm1, gp.smalle, gp.smallp, 1.0 / gp.gm1};
     const Gas<TC> gc{(TC)gp.gamma, (TC)gp.gm1, (TC)gp.smalle, (TC)gp.smallp, (TC)(1.0 / gp.gm1)};
     const int slope_type = gp.slope_type, interpol_type = gp.interpol_type;
-    const double dt = *d_dt;
+    double dt;
+    step_of(lv, p, d_dt, dt, dx, inv_dx);
     const double dtdx = dt / dx;
     const TC dtc = (TC)dt, dxc = (TC)dx;
+    const Len<TC, EXACT> len{dxc, (TC)inv_dx};
     const int cC = cell_id(m, lv.grid[p], ic0);
 
     double uC[5];
@@ -549,8 +627,8 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
         sel5(par, qS, qX, qL1);
         sel5(par, qX, qS, qR1);
         TC OCm[5], OCp[5], GCm[5], GCp[5];
-        own_trace(qL1, qC, qR1, d, slope_type, dtc, dxc, OCm, OCp, gc);
-        ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, dxc, GCm, GCp, gc);
+        own_trace(qL1, qC, qR1, d, slope_type, dtc, len, OCm, OCp, gc);
+        ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, len, GCm, GCp, gc);
         TC GSm[5], GSp[5];
         shfl5(GCm, GSm, b);
         shfl5(GCp, GSp, b);
@@ -582,8 +660,8 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
                 sel5(par, qC, qX2, a5);
                 sel5(par, qX2, qC, c5);
                 TC GXm[5], GXp[5], OXm[5], OXp[5];
-                ghost_trace(a5, qX, c5, d, interpol_type, dtc, dxc, GXm, GXp, gc);
-                own_trace(a5, qX, c5, d, slope_type, dtc, dxc, OXm, OXp, gc);
+                ghost_trace(a5, qX, c5, d, interpol_type, dtc, len, GXm, GXp, gc);
+                own_trace(a5, qX, c5, d, slope_type, dtc, len, OXm, OXp, gc);
                 TC ql[5], qr[5];
                 sel5(par, OCm, GXm, ql);
                 sel5(par, GXp, OCp, qr);
@@ -619,7 +697,7 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
             prim(rc.y, qX, qnl);
             prim(rc.z, qX, qnr);
             TC Gm[5], Gp[5];
-            ghost_trace(qnl, qX, qnr, d, interpol_type, dtc, dxc, Gm, Gp, gc);
+            ghost_trace(qnl, qX, qnr, d, interpol_type, dtc, len, Gm, Gp, gc);
             TC ql[5], qr[5];
             sel5(par, OCm, Gm, ql);
             sel5(par, Gp, OCp, qr);
@@ -798,36 +876,66 @@ hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, doubl
     return hipGetLastError();
 }
 
+hipError_t launch_courant_boxes(const MeshDev& m, const LevelDev& lv, double gamma, double courant_factor,
+                                unsigned long long* keys, hipStream_t stream) {
+    if (lv.noct == 0) return hipSuccess;
+    if (!lv.obox || !lv.box_len) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_courant_boxes, dim3(blocks_for(lv.noct)), dim3(BLOCK), 0, stream, m, lv, gamma,
+                       courant_factor, keys);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_keys(unsigned long long* keys, int n, double value, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_keys, dim3((n + 255) / 256), dim3(256), 0, stream, keys, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_keys_to_dt(const unsigned long long* keys, int n, double* dt, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_keys_to_dt, dim3((n + 255) / 256), dim3(256), 0, stream, keys, n, dt);
+    return hipGetLastError();
+}
+
+namespace {
+
+template <int R, bool MIXED, bool EXACT>
+void launch_form(bool oct, dim3 grid, dim3 block, const MeshDev& m, const LevelDev& lv, const GasParams& gp,
+                 const double* d_dt, double dx, double inv_dx, hipStream_t stream) {
+    if (oct)
+        hipLaunchKernelGGL((k_godunov_oct<R, MIXED, EXACT>), grid, block, 0, stream, m, lv, gp, d_dt, dx, inv_dx);
+    else
+        hipLaunchKernelGGL((k_godunov<R, MIXED, EXACT>), grid, block, 0, stream, m, lv, gp, d_dt, dx, inv_dx);
+}
+
+// The FP32 half of the mixed form divides through dv() whatever dx is, so
+// only the FP64 form has an exact-reciprocal instantiation.
+template <int R>
+void launch_riemann(bool oct, bool mixed, bool exact, dim3 grid, dim3 block, const MeshDev& m, const LevelDev& lv,
+                    const GasParams& gp, const double* d_dt, double dx, double inv_dx, hipStream_t stream) {
+    if (mixed) launch_form<R, true, false>(oct, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream);
+    else if (exact) launch_form<R, false, true>(oct, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream);
+    else launch_form<R, false, false>(oct, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream);
+}
+
+}  // namespace
+
 hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams& gp, int riemann, int precision,
-                          int variant, const double* d_dt, double dx, hipStream_t stream) {
+                          int variant, const double* d_dt, double dx, double inv_dx, hipStream_t stream) {
     if (lv.noct == 0) return hipSuccess;
     if (precision != 0 && precision != 1) return hipErrorNotSupported;
     const bool mixed = precision == 1;
     const bool oct = variant == 1;
+    const bool exact = inv_dx != 0.0;
     dim3 grid(oct ? (lv.noct + O_OCTS - 1) / O_OCTS : (lv.noct + G_OCTS - 1) / G_OCTS);
     dim3 block(oct ? O_BLOCK : G_BLOCK);
-#define RH_LAUNCH(R)                                                                                   \
-    do {                                                                                               \
-        if (oct) {                                                                                     \
-            if (mixed)                                                                                 \
-                hipLaunchKernelGGL((k_godunov_oct<R, true>), grid, block, 0, stream, m, lv, gp, d_dt, dx);  \
-            else                                                                                       \
-                hipLaunchKernelGGL((k_godunov_oct<R, false>), grid, block, 0, stream, m, lv, gp, d_dt, dx); \
-        } else {                                                                                       \
-            if (mixed)                                                                                 \
-                hipLaunchKernelGGL((k_godunov<R, true>), grid, block, 0, stream, m, lv, gp, d_dt, dx);      \
-            else                                                                                       \
-                hipLaunchKernelGGL((k_godunov<R, false>), grid, block, 0, stream, m, lv, gp, d_dt, dx);     \
-        }                                                                                              \
-    } while (0)
     switch (riemann) {
-        case 0: RH_LAUNCH(0); break;
-        case 1: RH_LAUNCH(1); break;
-        case 2: RH_LAUNCH(2); break;
-        case 4: RH_LAUNCH(4); break;
+        case 0: launch_riemann<0>(oct, mixed, exact, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream); break;
+        case 1: launch_riemann<1>(oct, mixed, exact, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream); break;
+        case 2: launch_riemann<2>(oct, mixed, exact, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream); break;
+        case 4: launch_riemann<4>(oct, mixed, exact, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream); break;
         default: return hipErrorNotSupported;
     }
-#undef RH_LAUNCH
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || lv.ntgt == 0) return e;
     hipLaunchKernelGGL(k_coarse_reflux, dim3((lv.ntgt + 255) / 256), dim3(256), 0, stream, m, lv);

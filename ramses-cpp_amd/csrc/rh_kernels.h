@@ -27,6 +27,12 @@ struct LevelDev {
     int* tgt_off = nullptr;     // [ntgt+1]
     int* tgt_slot = nullptr;    // [nslots used]
     double* slot_r = nullptr;   // [5*nslots] coarse reflux contributions
+    // Ensemble form: the box of each oct and, per box, its cell width dx
+    // (box_len[b]) and exact reciprocal (box_len[nbox+b], 0 when dx is not a
+    // power of two).  Null for a single mesh, whose dx is a kernel argument.
+    int nbox = 0;
+    int* obox = nullptr;              // [noct]
+    const double* box_len = nullptr;  // [2*nbox]
 };
 
 // Geometry / state of the bound mesh on the device.
@@ -53,8 +59,20 @@ int courant_blocks(const LevelDev& lv);
 // Godunov update of one level; dt is read from device memory (*d_dt).
 // variant 0: one thread per cell, octs strided across lanes; variant 1: the
 // eight cells of an oct in adjacent lanes, sharing by cross-lane shuffles.
+// dt is d_dt[0] for a single mesh and d_dt[box] for an ensemble.  inv_dx is
+// 1/dx when dx is a power of two (the FP64 form then multiplies by it, with
+// identical results), else 0; an ensemble reads dx and 1/dx per box and
+// passes inv_dx != 0 when every box has an exact reciprocal.
 hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams& gp, int riemann, int precision,
-                          int variant, const double* d_dt, double dx, hipStream_t stream);
+                          int variant, const double* d_dt, double dx, double inv_dx, hipStream_t stream);
+// Ensemble Courant step: keys[b] <- min(keys[b], key(dt candidate)) over the
+// leaf cells of box b (keys ordered like the doubles they encode, NaN last).
+hipError_t launch_courant_boxes(const MeshDev& m, const LevelDev& lv, double gamma, double courant_factor,
+                                unsigned long long* keys, hipStream_t stream);
+// keys[0..n) <- key(value).
+hipError_t launch_fill_keys(unsigned long long* keys, int n, double value, hipStream_t stream);
+// dt[i] <- the double encoded by keys[i], i < n.
+hipError_t launch_keys_to_dt(const unsigned long long* keys, int n, double* dt, hipStream_t stream);
 hipError_t launch_interpol(int n_octs, const double* d_u1, double* d_u2, const GasParams& gp, int interpol_var,
                            hipStream_t stream);
 

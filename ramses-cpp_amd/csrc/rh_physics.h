@@ -38,6 +38,22 @@ RH_DEV float ssqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 RH_DEV double dv(double a, double b) { return a / b; }
 RH_DEV float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// Cell width of the slope and trace divisions.  When dx is a power of two
+// (EXACT_INV) its reciprocal is exact, and a*(1/dx) rounds the same real
+// number as a/dx once, so the FP64 form multiplies instead of dividing and
+// stays bit-identical.  Otherwise, and in the FP32 half of the mixed form,
+// over() is dv().
+template <typename T, bool EXACT_INV> struct Len {
+    T dx, inv;
+};
+template <typename T, bool E> RH_DEV T over(T a, const Len<T, E>& l) {
+    if constexpr (E && sizeof(T) == 8) return a * l.inv;
+    else return dv(a, l.dx);
+}
+// The coarse cell width 2*dx of the ghost-state interpolation (its
+// reciprocal 0.5/dx stays exact).
+template <typename T, bool E> RH_DEV Len<T, E> coarse(const Len<T, E>& l) { return {T(2.0) * l.dx, T(0.5) * l.inv}; }
+
 template <typename T> RH_DEV void swap_(T& a, T& b) {
     T t = a;
     a = b;
@@ -165,9 +181,9 @@ RH_DEV void trace(const T q[5], const T dq[5], T dt, T dx, T qm[5], T qp[5], con
 // Own face states of a cell along dimension d (HydroSolver.cpp:93-115 with
 // compute_slopes :653-678).  ql/qc/qr are the unrotated primitives of the
 // left neighbour, the cell and the right neighbour; outputs are rotated.
-template <typename T>
-RH_DEV void own_trace(const T ql[5], const T qc[5], const T qr[5], int d, int slope_type, T dt, T dx, T qm[5],
-                      T qp[5], const Gas<T>& g) {
+template <typename T, bool E>
+RH_DEV void own_trace(const T ql[5], const T qc[5], const T qr[5], int d, int slope_type, T dt, const Len<T, E>& len,
+                      T qm[5], T qp[5], const Gas<T>& g) {
     T a[5], b[5], c[5];
 #pragma unroll
     for (int iv = 0; iv < 5; ++iv) {
@@ -183,22 +199,22 @@ RH_DEV void own_trace(const T ql[5], const T qc[5], const T qr[5], int d, int sl
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) dq[iv] = T(0.0);
     } else {
-        T nu = dv(b[1] * dt, dx);
+        T nu = over(b[1] * dt, len);
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
             if ((slope_type == 5 || slope_type == 6) && iv != 0) dq[iv] = T(0.0);
-            else dq[iv] = dv(limiter(a[iv], b[iv], c[iv], slope_type, nu), dx);
+            else dq[iv] = over(limiter(a[iv], b[iv], c[iv], slope_type, nu), len);
         }
     }
-    trace(b, dq, dt, dx, qm, qp, g);
+    trace(b, dq, dt, len.dx, qm, qp, g);
 }
 
 // Ghost face states of a neighbour cell n along dimension d, interpolated
 // with interpol_type slopes over 2*dx (HydroSolver.cpp:357-440).  Inputs are
 // unrotated primitives of n's left neighbour, n, n's right neighbour.
-template <typename T>
-RH_DEV void ghost_trace(const T ql[5], const T qc[5], const T qr[5], int d, int interpol_type, T dt, T dx,
-                        T qm[5], T qp[5], const Gas<T>& g) {
+template <typename T, bool E>
+RH_DEV void ghost_trace(const T ql[5], const T qc[5], const T qr[5], int d, int interpol_type, T dt,
+                        const Len<T, E>& len, T qm[5], T qp[5], const Gas<T>& g) {
     T a[5], b[5], c[5];
 #pragma unroll
     for (int iv = 0; iv < 5; ++iv) {
@@ -209,11 +225,11 @@ RH_DEV void ghost_trace(const T ql[5], const T qc[5], const T qr[5], int d, int 
     rotate(a, d);
     rotate(b, d);
     rotate(c, d);
-    T dx_coarse = T(2.0) * dx;
+    const Len<T, E> lc = coarse(len);
     T dq[5];
 #pragma unroll
-    for (int iv = 0; iv < 5; ++iv) dq[iv] = dv(ghost_limiter(b[iv] - a[iv], c[iv] - b[iv], interpol_type), dx_coarse);
-    trace(b, dq, dt, dx_coarse, qm, qp, g);
+    for (int iv = 0; iv < 5; ++iv) dq[iv] = over(ghost_limiter(b[iv] - a[iv], c[iv] - b[iv], interpol_type), lc);
+    trace(b, dq, dt, lc.dx, qm, qp, g);
 }
 
 // Ghost of a boundary face: the cell's own state, normal momentum flipped for

@@ -95,6 +95,10 @@ def lib() -> C.CDLL:
         L.rh_set_kernel_variant.argtypes = [C.c_void_p, C.c_int]
         L.rh_get_kernel_variant.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.rh_autotune.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int), _DP]
+        L.rh_init_shock_tube.argtypes = [C.POINTER(RhMesh), _DP, C.c_double, C.c_double, _DP, _DP, _DP, _DP]
+        L.rh_bind_ensemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhMesh), C.c_int]
+        L.rh_run_ensemble.argtypes = [C.c_void_p, _IP, _DP, C.c_double, C.c_int, _DP]
+        L.rh_ensemble_leaf_cells.argtypes = [C.c_void_p, _IP, C.POINTER(C.c_longlong)]
         _lib = L
     return _lib
 
@@ -103,7 +107,8 @@ EXPORTED = [
     "rh_create", "rh_destroy", "rh_bind", "rh_set_unew", "rh_godunov_fine", "rh_set_uold", "rh_courant",
     "rh_interpol_hydro", "rh_sync", "rh_last_error", "rh_run_uniform", "rh_uniform_ngrid", "rh_build_uniform",
     "rh_init_regions", "rh_init_lognormal", "rh_timers_enable", "rh_timers_read", "rh_timers_reset",
-    "rh_level_leaf_cells", "rh_set_kernel_variant", "rh_get_kernel_variant", "rh_autotune",
+    "rh_level_leaf_cells", "rh_set_kernel_variant", "rh_get_kernel_variant", "rh_autotune", "rh_init_shock_tube",
+    "rh_bind_ensemble", "rh_run_ensemble", "rh_ensemble_leaf_cells",
 ]
 
 
@@ -239,6 +244,66 @@ def sedov_regions(levelmin: int, nx: int = 1, gamma: float = 1.4, energy: float 
     ]
 
 
+def init_shock_tube(m: Mesh, normal, point, left, right, gamma: float, boxlen: float = 1.0) -> None:
+    """Two primitive states (d, u, v, w, p) split by the plane through
+    `point` with normal `normal` (rh_init_shock_tube)."""
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (normal, point, left, right)]
+    mm = m.rh_mesh()
+    st = lib().rh_init_shock_tube(C.byref(mm), _dp(m.xg), boxlen, gamma, *[_dp(a) for a in arrs])
+    if st != RH_OK:
+        raise RuntimeError(f"rh_init_shock_tube failed ({st})")
+
+
+# Box sides of the shock-tube ensemble: nx * 2^(L-1) cells per dimension with
+# nx coarse cells (nx in 1, 3, 5, 7) and reference level L >= 2.
+def _quantised_sides(lo: int, hi: int) -> list[tuple[int, int, int]]:
+    out = []
+    for nx in (1, 3, 5, 7):
+        for L in range(2, 16):
+            side = nx * 2 ** (L - 1)
+            if lo <= side <= hi:
+                out.append((side, nx, L))
+    return sorted(out)
+
+
+@dataclass
+class EnsembleBox:
+    mesh: Mesh
+    nx: int
+    level: int
+    dx: float
+    normal: np.ndarray
+    left: np.ndarray
+    right: np.ndarray
+
+
+def shock_tube_ensemble(nbox: int = 96, seed: int = 20261202, side_min: int = 16, side_max: int = 64,
+                        gamma: float = 1.4, boxlen: float = 1.0) -> list[EnsembleBox]:
+    """Independent periodic cubic boxes, each a shock tube through its centre.
+    Sides are drawn log-uniformly in [side_min, side_max] from
+    numpy.random.default_rng(seed) and rounded (in log) to the nearest
+    nx * 2^(L-1); the plane normal is uniform on the sphere; each side's
+    density and pressure are uniform in [0.1, 1], at rest.  The solver's cell
+    width is dx = boxlen / (nx * 2^L), as the reference passes it
+    (Simulation.cpp:548)."""
+    rng = np.random.default_rng(seed)
+    table = _quantised_sides(side_min, side_max)
+    logs = np.log([t[0] for t in table])
+    boxes = []
+    for _ in range(nbox):
+        s = rng.uniform(np.log(side_min), np.log(side_max))
+        side, nx, L = table[int(np.argmin(np.abs(logs - s)))]
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        dl, pl, dr, pr = rng.uniform(0.1, 1.0, 4)
+        left = np.array([dl, 0.0, 0.0, 0.0, pl])
+        right = np.array([dr, 0.0, 0.0, 0.0, pr])
+        m = uniform_mesh(L, nx=nx, ny=nx, nz=nx)
+        init_shock_tube(m, n, np.full(3, 0.5 * boxlen), left, right, gamma, boxlen)
+        boxes.append(EnsembleBox(m, nx, L, boxlen / (nx * 2 ** L), n, left, right))
+    return boxes
+
+
 class RhError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"rhydro error {code}: {msg}")
@@ -270,6 +335,29 @@ class Solver:
         self.mesh, self._f = mesh, f
         self._rhm = mesh.rh_mesh(f)
         self._chk(lib().rh_bind(self._ctx, C.byref(self._rhm), mode))
+
+    def bind_ensemble(self, meshes: list, mode=RH_DEVICE_RESIDENT):
+        """Binds independent meshes as one ensemble (rh_bind_ensemble)."""
+        self.mesh, self._f = list(meshes), None
+        arr = (RhMesh * len(meshes))(*[m.rh_mesh() for m in meshes])
+        self._rhm = arr
+        self._chk(lib().rh_bind_ensemble(self._ctx, len(meshes), arr, mode))
+
+    def run_ensemble(self, levels, dx, courant_factor, nsteps) -> np.ndarray:
+        """Advances every box nsteps steps; returns the time steps, shape
+        (nsteps, nbox)."""
+        lv = np.ascontiguousarray(levels, np.int32)
+        dxa = np.ascontiguousarray(dx, np.float64)
+        nbox = len(lv)
+        dts = np.zeros(max(nsteps, 1) * nbox, np.float64)
+        self._chk(lib().rh_run_ensemble(self._ctx, _ip(lv), _dp(dxa), courant_factor, nsteps, _dp(dts)))
+        return dts[: nsteps * nbox].reshape(nsteps, nbox)
+
+    def ensemble_leaf_cells(self, levels) -> int:
+        lv = np.ascontiguousarray(levels, np.int32)
+        n = C.c_longlong()
+        self._chk(lib().rh_ensemble_leaf_cells(self._ctx, _ip(lv), C.byref(n)))
+        return n.value
 
     def set_unew(self, ilevel):
         self._chk(lib().rh_set_unew(self._ctx, ilevel))

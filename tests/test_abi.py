@@ -2,6 +2,7 @@
 the public headers declare; argument validation happens before any device
 call."""
 import ctypes as C
+import glob
 import os
 import re
 
@@ -14,8 +15,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     names = set()
-    for h in ("rhydro.h", "rhydro_driver.h"):
-        text = open(os.path.join(ROOT, "include", h)).read()
+    for h in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))):
+        text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names |= set(re.findall(r"\b(rh_[a-z_0-9]+)\s*\(", text))
     return names
