@@ -126,6 +126,51 @@ def measure_ensemble(args, rhydro, np):
     }
 
 
+def measure_sedov64(args, rhydro, np):
+    """The 64^3 Sedov blast (reference level 7, 262,144 cells; p = 1e-5
+    outside, the unit energy in the 8 central cells), gamma = 1.4, HLLC,
+    MinMod, 20 steps of the FP64 form from the initial state, timed on the
+    GPU and, unless --no-cpu, through the oracle on one host core; the two
+    final states are compared value for value."""
+    level, nsteps, cf = 7, 20, 0.8
+    mesh = rhydro.uniform_mesh(level)
+    rhydro.init_regions(mesh, rhydro.sedov_regions(level), 1.4)
+    u0 = mesh.uold.copy()
+    dx = 1.0 / 2 ** level
+    s = rhydro.Solver(gamma=1.4, slope_type=1, riemann="hllc")
+    s.bind(mesh, rhydro.RH_DEVICE_RESIDENT)
+    nleaf = s.leaf_cells(level)
+    s.run_uniform(level, dx, cf, 1)  # first use: tables and kernels loaded
+    mesh.uold[:] = u0
+    mesh.unew[:] = u0
+    s.bind(mesh, rhydro.RH_DEVICE_RESIDENT)
+    t0 = time.perf_counter()
+    dts = s.run_uniform(level, dx, cf, nsteps)
+    wall = time.perf_counter() - t0
+    s.sync_to_host()
+    s.close()
+    out = {"workload": "Sedov blast 64^3 (reference level 7), gamma=1.4, HLLC, slope_type 1, 20 steps, FP64 form",
+           "cells": int(nleaf), "steps": nsteps, "gpu_value": nleaf * nsteps / wall, "gpu_wall_s": wall,
+           "dt_first": float(dts[0]), "dt_last": float(dts[-1])}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        ref = rhydro.uniform_mesh(level)
+        ref.uold[:] = u0
+        ref.unew[:] = u0
+        om = oracle.OracleMesh(ref)
+        tc = time.perf_counter()
+        dto = om.run_uniform(oracle.params(1.4, 1, 2), level, dx, cf, nsteps)
+        tcpu = time.perf_counter() - tc
+        out.update({"cpu_value": nleaf * nsteps / tcpu, "cpu_wall_s": tcpu, "cpu_kind": "port", "cpu_cores": 1,
+                    "identical_to_oracle": bool(np.array_equal(dts, dto) and np.array_equal(mesh.uold, ref.uold))})
+    print(f"[bench] sedov64: gpu {wall:.4f}s" + (f", cpu {out['cpu_wall_s']:.1f}s, identical "
+                                                 f"{out['identical_to_oracle']}" if "cpu_wall_s" in out else ""),
+          file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +184,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--boxes", type=int, default=96, help="boxes of the shock-tube ensemble (0: skip it)")
+    ap.add_argument("--no-variants", action="store_true", help="skip the HLL / slope-2 variants of the workload")
+    ap.add_argument("--no-sedov", action="store_true", help="skip the 64^3 Sedov run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -162,11 +209,11 @@ def main() -> int:
     cf = 0.8
     u0 = mesh.uold.copy()
 
-    def measure(precision):
+    def measure(precision, riemann=args.riemann, slope=args.slope):
         """Warmup, then exactly `steps` timed steps of one precision form."""
         mesh.uold[:] = u0
         mesh.unew[:] = u0
-        s = rhydro.Solver(gamma=gamma, slope_type=args.slope, riemann=args.riemann, precision=precision)
+        s = rhydro.Solver(gamma=gamma, slope_type=slope, riemann=riemann, precision=precision)
         s.bind(mesh, rhydro.RH_DEVICE_RESIDENT)
         nleaf = s.leaf_cells(level)
         warm_dts = s.run_uniform(level, dx, cf, max(args.warmup, 1))  # ends with a stream synchronisation
@@ -241,7 +288,25 @@ def main() -> int:
                           f"{args.warmup}+{args.steps} steps with each form's own time steps",
         }
 
+    # The other Riemann solver / slope combinations of the same workload:
+    # godunov kernel time and roofline fraction per form.
+    variants = []
+    if not args.no_variants:
+        for rv, sv in (("hllc", 1), ("hllc", 2), ("hll", 1), ("hll", 2)):
+            if (rv, sv) == (args.riemann, args.slope):
+                a, b = fp64, mixed
+            else:
+                a = measure(rhydro.RH_PREC_FP64, rv, sv)
+                b = measure(rhydro.RH_PREC_MIXED, rv, sv) if not args.fp64_only else None
+            row = {"riemann": rv, "slope_type": sv, "fp64_value": nleaf * args.steps / a["wall"],
+                   "fp64_godunov_ms": a["kernel_ms"]["godunov"], "fp64_roofline_frac": a["achieved"] / HBM_PEAK_GBS}
+            if b is not None:
+                row.update({"mixed_value": nleaf * args.steps / b["wall"], "mixed_godunov_ms": b["kernel_ms"]["godunov"],
+                            "mixed_roofline_frac": b["achieved"] / HBM_PEAK_GBS})
+            variants.append(row)
+
     ens_info = measure_ensemble(args, rhydro, np) if args.boxes > 0 else None
+    sedov_info = measure_sedov64(args, rhydro, np) if not args.no_sedov else None
 
     value = nleaf * args.steps / fp64["wall"]
     achieved = fp64["achieved"]
@@ -302,7 +367,9 @@ def main() -> int:
             "kernel_ms": fp64["kernel_ms"],
             "autotune": fp64["autotune"],
             "mixed": mixed_info,
+            "variants": variants,
             "ensemble": ens_info,
+            "sedov64": sedov_info,
         },
         "roofline": {
             "bound": "hbm",

@@ -124,6 +124,18 @@ int rh_courant(rh_ctx* ctx, int ilevel, double dx, double gamma, double courant_
  * [n_octs][7][64], u2 is [n_octs][8][64]). */
 int rh_interpol_hydro(rh_ctx* ctx, int n_octs, const double* u1, double* u2);
 
+/* Replaces HydroSolver::add_gravity_source_terms(ilevel, dt)
+ * (HydroSolver.cpp:980-998): for every leaf cell of the level,
+ * unew momentum += rho*f*dt (clamped to +-10 rho) and unew energy +=
+ * uold momentum . f dt + rho f^2 dt^2 / 2, with the bound f (zero when the
+ * mesh has no f, as AmrGrid::allocate leaves f_vec). */
+int rh_add_gravity_source_terms(rh_ctx* ctx, int ilevel, double dt);
+
+/* Replaces HydroSolver::synchro_hydro_fine(ilevel, dt)
+ * (HydroSolver.cpp:1000-1043, nener = 0, no barotropic EOS): the same kick
+ * applied to uold at constant internal energy (gamma from rh_params). */
+int rh_synchro_hydro_fine(rh_ctx* ctx, int ilevel, double dt);
+
 /* Explicit host<->device copies of uold and/or unew (what = RH_UOLD|RH_UNEW).
  * No reference counterpart (the reference state lives on the host). */
 int rh_sync(rh_ctx* ctx, int direction, int what);

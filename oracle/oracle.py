@@ -62,6 +62,8 @@ def lib() -> C.CDLL:
         L.or_set_uold.argtypes = [C.POINTER(OrMesh), C.c_int]
         L.or_courant.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.or_courant.restype = C.c_double
+        L.or_add_gravity_source_terms.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double]
+        L.or_synchro_hydro_fine.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double, C.c_double]
         L.or_interpol_hydro.argtypes = [C.POINTER(OrParams), C.c_int, _DP, _DP]
         L.or_ctoprim.argtypes = [_DP, _DP, C.c_double]
         L.or_trace.argtypes = [_DP, _DP, C.c_double, C.c_double, _DP, _DP, C.c_double]
@@ -145,6 +147,12 @@ class OracleMesh:
 
     def set_uold(self, ilevel):
         lib().or_set_uold(C.byref(self.s), ilevel)
+
+    def add_gravity_source_terms(self, ilevel, dt):
+        lib().or_add_gravity_source_terms(C.byref(self.s), ilevel, dt)
+
+    def synchro_hydro_fine(self, ilevel, dt, gamma):
+        lib().or_synchro_hydro_fine(C.byref(self.s), ilevel, dt, gamma)
 
     def courant(self, ilevel, dx, gamma, courant_factor, boxlen=1.0) -> float:
         return lib().or_courant(C.byref(self.s), ilevel, dx, gamma, courant_factor, boxlen)

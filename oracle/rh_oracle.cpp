@@ -889,6 +889,62 @@ void or_set_uold(const or_mesh* m, int ilevel) {
     }
 }
 
+// HydroSolver.cpp:980-998: gravity kick of the level's leaf cells into
+// unew (f_vec zero when m->f is null, as AmrGrid::allocate leaves it).
+void or_add_gravity_source_terms(const or_mesh* m, int ilevel, double dt) {
+    Ctx c{m, nullptr};
+    auto f = [&](int idc, int idim) { return m->f ? m->f[(size_t)(idim - 1) * m->ncell + (idc - 1)] : 0.0; };
+    int ig = c.headl(m->myid, ilevel);
+    while (ig > 0) {
+        for (int ic = 1; ic <= 8; ++ic) {
+            int idc = m->ncoarse + (ic - 1) * m->ngridmax + ig;
+            if (m->son[idc - 1] > 0) continue;
+            double d = sclamp(c.uold(idc, 1), 1e-10, 1e6);
+            for (int idim = 1; idim <= 3; ++idim) {
+                double fv = f(idc, idim);
+                c.unew(idc, 1 + idim) += d * fv * dt;
+                c.unew(idc, 1 + idim) = sclamp(c.unew(idc, 1 + idim), -d * 10.0, d * 10.0);
+                c.unew(idc, 5) += c.uold(idc, 1 + idim) * fv * dt + 0.5 * d * fv * fv * dt * dt;
+            }
+        }
+        ig = m->next[ig - 1];
+    }
+}
+
+// HydroSolver.cpp:1000-1043 (nener=0, barotropic EOS off): gravity kick of
+// the level's leaf cells in uold, keeping their internal energy.
+void or_synchro_hydro_fine(const or_mesh* m, int ilevel, double dt, double gamma) {
+    Ctx c{m, nullptr};
+    auto f = [&](int idc, int idim) { return m->f ? m->f[(size_t)(idim - 1) * m->ncell + (idc - 1)] : 0.0; };
+    auto U = [&](int idc, int iv) -> double& { return m->uold[(size_t)(iv - 1) * m->ncell + idc - 1]; };
+    int ig = c.headl(m->myid, ilevel);
+    while (ig > 0) {
+        for (int ic = 1; ic <= 8; ++ic) {
+            int idc = m->ncoarse + (ic - 1) * m->ngridmax + ig;
+            if (m->son[idc - 1] > 0) continue;
+            double d = sclamp(U(idc, 1), 1e-10, 1e6);
+            double v2_old = 0;
+            for (int idim = 1; idim <= 3; ++idim) {
+                double v = U(idc, 1 + idim) / d;
+                v2_old += v * v;
+            }
+            double e_nonthermal = 0.0;
+            double e_int = smax(U(idc, 5) - 0.5 * d * v2_old - e_nonthermal, d * 1e-10 / (gamma - 1.0));
+            for (int idim = 1; idim <= 3; ++idim) {
+                U(idc, 1 + idim) += d * f(idc, idim) * dt;
+                U(idc, 1 + idim) = sclamp(U(idc, 1 + idim), -d * 10.0, d * 10.0);
+            }
+            double v2_new = 0;
+            for (int idim = 1; idim <= 3; ++idim) {
+                double v = U(idc, 1 + idim) / d;
+                v2_new += v * v;
+            }
+            U(idc, 5) = e_int + 0.5 * d * v2_new + e_nonthermal;
+        }
+        ig = m->next[ig - 1];
+    }
+}
+
 // HydroSolver.cpp:878-977 (barotropic_eos off, nener=0)
 double or_courant(const or_mesh* m, int ilevel, double dx, double gamma, double courant_factor, double boxlen) {
     Ctx c{m, nullptr};

@@ -3,9 +3,9 @@
 // Copy this file into the reference's patch/ directory (globbed into the
 // library by CMakeLists.txt:47-53), add -DRAMSES_HAS_PATCH_HYDRO so the
 // default factory (SolverFactory.cpp:5-9) steps aside, and link librhydro.so.
-// Every hydro virtual of HydroSolver.hpp:19-29 that lies on the Godunov path
-// is forwarded to the matching C-ABI call of include/rhydro.h; gravity source
-// terms, synchro and diagnostics keep the base-class host code.
+// Every hydro virtual of HydroSolver.hpp:19-29 except get_diagnostics (an
+// empty function in the reference) is forwarded to the matching C-ABI call of
+// include/rhydro.h.
 //
 // The reference mutates the AmrGrid on the host between calls (refinement,
 // restriction, flagging), so the context runs in RH_HOST_COHERENT mode: state
@@ -52,6 +52,16 @@ public:
         double dt = 0.0;
         check(rh_courant(ctx_, ilevel, dx, gamma, courant_factor, &dt), "rh_courant");
         return dt;
+    }
+
+    void add_gravity_source_terms(int ilevel, real_t dt) override {
+        ensure_bound();
+        check(rh_add_gravity_source_terms(ctx_, ilevel, dt), "rh_add_gravity_source_terms");
+    }
+
+    void synchro_hydro_fine(int ilevel, real_t dt) override {
+        ensure_bound();
+        check(rh_synchro_hydro_fine(ctx_, ilevel, dt), "rh_synchro_hydro_fine");
     }
 
     void interpol_hydro(const real_t u1[7][64], real_t u2[8][64]) override {

@@ -165,7 +165,13 @@ int check_bound(rh_ctx* c) {
 // In host-coherent mode the host may have changed uold/unew since the last
 // call: copy them in first.
 int pre_call(rh_ctx* c) {
-    if (c->mode == RH_HOST_COHERENT) return upload_state(c, RH_UOLD | RH_UNEW);
+    if (c->mode != RH_HOST_COHERENT) return RH_OK;
+    int st = upload_state(c, RH_UOLD | RH_UNEW);
+    if (st) return st;
+    // The host's gravity solver may have updated f since the last call.
+    if (c->d_f && c->mesh.f)
+        RH_HIP(c, hipMemcpyAsync(c->d_f, c->mesh.f, (size_t)3 * c->mesh.ncell * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
     return RH_OK;
 }
 
@@ -368,6 +374,28 @@ int rh_set_uold(rh_ctx* c, int ilevel) {
     Timed tm(c, TK_SET_UOLD);
     RH_HIP(c, rh::launch_set_uold(mesh_dev(c), *lv, c->stream));
     tm.done();
+    return post_call(c, RH_UOLD);
+}
+
+int rh_add_gravity_source_terms(rh_ctx* c, int ilevel, double dt) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    if ((st = pre_call(c))) return st;
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    RH_HIP(c, rh::launch_gravity_source(mesh_dev(c), *lv, dt, c->stream));
+    return post_call(c, RH_UNEW);
+}
+
+int rh_synchro_hydro_fine(rh_ctx* c, int ilevel, double dt) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    if ((st = pre_call(c))) return st;
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    RH_HIP(c, rh::launch_synchro(mesh_dev(c), *lv, dt, c->p.gamma, c->stream));
     return post_call(c, RH_UOLD);
 }
 

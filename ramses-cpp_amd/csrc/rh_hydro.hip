@@ -67,6 +67,65 @@ __global__ __launch_bounds__(BLOCK) void k_set_uold(MeshDev m, LevelDev lv) {
     for (int iv = 0; iv < 5; ++iv) m.uold[(size_t)iv * m.ncell + c] = m.unew[(size_t)iv * m.ncell + c];
 }
 
+RH_DEV double fval(const MeshDev& m, size_t c, int idim) {
+    return m.f ? m.f[(size_t)(idim - 1) * m.ncell + c] : 0.0;
+}
+
+// HydroSolver.cpp:980-998 per leaf cell of the level: gravity kick into unew.
+__global__ __launch_bounds__(BLOCK) void k_gravity_source(MeshDev m, LevelDev lv, double dt) {
+    const int t = threadIdx.x;
+    const int p = blockIdx.x * OCTS_PER_BLOCK + (t % OCTS_PER_BLOCK);
+    const int ic0 = t / OCTS_PER_BLOCK;
+    if (p >= lv.noct) return;
+    if (!((lv.leaf[p] >> ic0) & 1)) return;
+    const size_t c = (size_t)cell_id(m, lv.grid[p], ic0) - 1;
+    const double d = sclamp(m.uold[c], 1e-10, 1e6);
+    double e = m.unew[(size_t)4 * m.ncell + c];
+    for (int idim = 1; idim <= 3; ++idim) {
+        const double f = fval(m, c, idim);
+        const size_t k = (size_t)idim * m.ncell + c;
+        double mom = m.unew[k] + d * f * dt;
+        mom = sclamp(mom, -d * 10.0, d * 10.0);
+        m.unew[k] = mom;
+        e = e + (m.uold[k] * f * dt + 0.5 * d * f * f * dt * dt);
+    }
+    m.unew[(size_t)4 * m.ncell + c] = e;
+}
+
+// HydroSolver.cpp:1000-1043 per leaf cell of the level: gravity kick of uold
+// keeping the internal energy (nener = 0, no barotropic EOS).
+__global__ __launch_bounds__(BLOCK) void k_synchro(MeshDev m, LevelDev lv, double dt, double gamma) {
+    const int t = threadIdx.x;
+    const int p = blockIdx.x * OCTS_PER_BLOCK + (t % OCTS_PER_BLOCK);
+    const int ic0 = t / OCTS_PER_BLOCK;
+    if (p >= lv.noct) return;
+    if (!((lv.leaf[p] >> ic0) & 1)) return;
+    const size_t c = (size_t)cell_id(m, lv.grid[p], ic0) - 1;
+    double u[5];
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) u[iv] = m.uold[(size_t)iv * m.ncell + c];
+    const double d = sclamp(u[0], 1e-10, 1e6);
+    double v2_old = 0;
+    for (int idim = 1; idim <= 3; ++idim) {
+        const double v = u[idim] / d;
+        v2_old = v2_old + v * v;
+    }
+    const double e_nonthermal = 0.0;
+    const double e_int = smax(u[4] - 0.5 * d * v2_old - e_nonthermal, d * 1e-10 / (gamma - 1.0));
+    for (int idim = 1; idim <= 3; ++idim) {
+        u[idim] = u[idim] + d * fval(m, c, idim) * dt;
+        u[idim] = sclamp(u[idim], -d * 10.0, d * 10.0);
+    }
+    double v2_new = 0;
+    for (int idim = 1; idim <= 3; ++idim) {
+        const double v = u[idim] / d;
+        v2_new = v2_new + v * v;
+    }
+    u[4] = e_int + 0.5 * d * v2_new + e_nonthermal;
+#pragma unroll
+    for (int iv = 1; iv < 5; ++iv) m.uold[(size_t)iv * m.ncell + c] = u[iv];
+}
+
 // NaN-skipping minimum: a NaN candidate never replaces the running value,
 // as with std::min(dt_max, dt_cand) in the reference loop.
 RH_DEV double min_skip_nan(double a, double b) { return (b < a) ? b : a; }
@@ -862,6 +921,18 @@ hipError_t launch_set_unew(const MeshDev& m, const LevelDev& lv, hipStream_t str
 hipError_t launch_set_uold(const MeshDev& m, const LevelDev& lv, hipStream_t stream) {
     if (lv.noct == 0) return hipSuccess;
     hipLaunchKernelGGL(k_set_uold, dim3(blocks_for(lv.noct)), dim3(BLOCK), 0, stream, m, lv);
+    return hipGetLastError();
+}
+
+hipError_t launch_gravity_source(const MeshDev& m, const LevelDev& lv, double dt, hipStream_t stream) {
+    if (lv.noct == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gravity_source, dim3(blocks_for(lv.noct)), dim3(BLOCK), 0, stream, m, lv, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_synchro(const MeshDev& m, const LevelDev& lv, double dt, double gamma, hipStream_t stream) {
+    if (lv.noct == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_synchro, dim3(blocks_for(lv.noct)), dim3(BLOCK), 0, stream, m, lv, dt, gamma);
     return hipGetLastError();
 }
 

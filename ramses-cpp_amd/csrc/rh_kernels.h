@@ -52,6 +52,11 @@ struct GasParams {
 // Launchers (defined in rh_hydro.hip).  All enqueue on `stream`.
 hipError_t launch_set_unew(const MeshDev& m, const LevelDev& lv, hipStream_t stream);
 hipError_t launch_set_uold(const MeshDev& m, const LevelDev& lv, hipStream_t stream);
+// Gravity source terms of the level's leaf cells: the kick into unew
+// (add_gravity_source_terms) and the kick of uold at constant internal energy
+// (synchro_hydro_fine).  m.f null means f = 0.
+hipError_t launch_gravity_source(const MeshDev& m, const LevelDev& lv, double dt, hipStream_t stream);
+hipError_t launch_synchro(const MeshDev& m, const LevelDev& lv, double dt, double gamma, hipStream_t stream);
 // Courant minimum into *d_dt (device).  d_part has >= courant_blocks(lv) doubles.
 hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
                           double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream);

@@ -95,6 +95,8 @@ def lib() -> C.CDLL:
         L.rh_set_kernel_variant.argtypes = [C.c_void_p, C.c_int]
         L.rh_get_kernel_variant.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.rh_autotune.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int), _DP]
+        L.rh_add_gravity_source_terms.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.rh_synchro_hydro_fine.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.rh_init_shock_tube.argtypes = [C.POINTER(RhMesh), _DP, C.c_double, C.c_double, _DP, _DP, _DP, _DP]
         L.rh_bind_ensemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhMesh), C.c_int]
         L.rh_run_ensemble.argtypes = [C.c_void_p, _IP, _DP, C.c_double, C.c_int, _DP]
@@ -108,7 +110,8 @@ EXPORTED = [
     "rh_interpol_hydro", "rh_sync", "rh_last_error", "rh_run_uniform", "rh_uniform_ngrid", "rh_build_uniform",
     "rh_init_regions", "rh_init_lognormal", "rh_timers_enable", "rh_timers_read", "rh_timers_reset",
     "rh_level_leaf_cells", "rh_set_kernel_variant", "rh_get_kernel_variant", "rh_autotune", "rh_init_shock_tube",
-    "rh_bind_ensemble", "rh_run_ensemble", "rh_ensemble_leaf_cells",
+    "rh_bind_ensemble", "rh_run_ensemble", "rh_ensemble_leaf_cells", "rh_add_gravity_source_terms",
+    "rh_synchro_hydro_fine",
 ]
 
 
@@ -367,6 +370,12 @@ class Solver:
 
     def godunov_fine(self, ilevel, dt, dx):
         self._chk(lib().rh_godunov_fine(self._ctx, ilevel, dt, dx))
+
+    def add_gravity_source_terms(self, ilevel, dt):
+        self._chk(lib().rh_add_gravity_source_terms(self._ctx, ilevel, dt))
+
+    def synchro_hydro_fine(self, ilevel, dt):
+        self._chk(lib().rh_synchro_hydro_fine(self._ctx, ilevel, dt))
 
     def courant(self, ilevel, dx, gamma, courant_factor) -> float:
         out = C.c_double()
