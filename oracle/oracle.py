@@ -237,3 +237,114 @@ def refine(m, p: OrParams, ilevel: int, cells) -> int:
     m.headf = headf.value
     m.epoch += 1
     return n
+
+
+# ---------------------------------------------------------------------------
+# Adaptive run (rh_oracle_amr.cpp): the reference's tree updater and
+# subcycled time integration around the oracle's hydro path.
+
+class OrAmrConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ngridmax", C.c_int),
+        ("levelmin", C.c_int), ("levelmax", C.c_int),
+        ("boxlen", C.c_double), ("gamma", C.c_double), ("courant_factor", C.c_double),
+        ("err_grad_d", C.c_double), ("err_grad_p", C.c_double), ("err_grad_v", C.c_double),
+        ("nexpand", C.c_int), ("hydro", OrParams), ("ic_kind", C.c_int), ("nreg", C.c_int),
+        ("reg_type", _IP),
+        ("x_center", _DP), ("y_center", _DP), ("z_center", _DP),
+        ("length_x", _DP), ("length_y", _DP), ("length_z", _DP),
+        ("d_region", _DP), ("u_region", _DP), ("v_region", _DP), ("w_region", _DP), ("p_region", _DP),
+        ("kh_amp", C.c_double),
+    ]
+
+
+class OrAmrState(C.Structure):
+    _fields_ = [
+        ("ncoarse", C.c_int), ("ngridmax", C.c_int), ("ncell", C.c_int), ("nlevelmax", C.c_int),
+        ("father", _IP), ("son", _IP), ("nbor", _IP), ("next", _IP), ("headl", _IP), ("numbl", _IP),
+        ("flag1", _IP), ("xg", _DP), ("uold", _DP), ("unew", _DP),
+        ("t", C.c_double), ("nstep", C.c_int), ("dtnew", _DP), ("cell_updates", C.c_longlong),
+        ("status", C.c_int),
+    ]
+
+
+def _amr_lib():
+    L = lib()
+    if not getattr(L, "_amr_ready", False):
+        L.or_amr_create.argtypes = [C.POINTER(OrAmrConfig)]
+        L.or_amr_create.restype = C.c_void_p
+        L.or_amr_step.argtypes = [C.c_void_p, C.c_int]
+        L.or_amr_get.argtypes = [C.c_void_p, C.POINTER(OrAmrState)]
+        L.or_amr_destroy.argtypes = [C.c_void_p]
+        L._amr_ready = True
+    return L
+
+
+class OracleAmr:
+    """An adaptive run of the oracle: initialize on creation, then coarse
+    steps.  `regions` as rhydro.init_regions takes them; ic='kh' selects the
+    Kelvin-Helmholtz slab."""
+
+    def __init__(self, nx=1, levelmin=4, levelmax=6, ngridmax=20000, gamma=1.4, courant_factor=0.8,
+                 err_grad_d=-1.0, err_grad_p=-1.0, err_grad_v=-1.0, nexpand=1, slope_type=1, riemann=2,
+                 interpol_type=1, interpol_var=0, boxlen=1.0, regions=None, ic="regions", kh_amp=0.01):
+        regions = regions or []
+        self._keep = []
+
+        def col(key, default):
+            a = np.array([float(r.get(key, default)) for r in regions] or [0.0], np.float64)
+            self._keep.append(a)
+            return a.ctypes.data_as(_DP)
+
+        types = np.array([1 if r.get("type", "cube") == "sphere" else 0 for r in regions] or [0], np.int32)
+        self._keep.append(types)
+        cfg = OrAmrConfig()
+        cfg.nx = cfg.ny = cfg.nz = nx
+        cfg.ngridmax, cfg.levelmin, cfg.levelmax = ngridmax, levelmin, levelmax
+        cfg.boxlen, cfg.gamma, cfg.courant_factor = boxlen, gamma, courant_factor
+        cfg.err_grad_d, cfg.err_grad_p, cfg.err_grad_v = err_grad_d, err_grad_p, err_grad_v
+        cfg.nexpand = nexpand
+        cfg.hydro = OrParams(gamma, slope_type, riemann, interpol_type, interpol_var, boxlen)
+        cfg.ic_kind = 1 if ic == "kh" else 0
+        cfg.nreg = len(regions)
+        cfg.reg_type = types.ctypes.data_as(_IP)
+        (cfg.x_center, cfg.y_center, cfg.z_center, cfg.length_x, cfg.length_y, cfg.length_z, cfg.d_region,
+         cfg.u_region, cfg.v_region, cfg.w_region, cfg.p_region) = (
+            col("x_center", 0.5), col("y_center", 0.5), col("z_center", 0.5), col("length_x", 1.0),
+            col("length_y", 1.0), col("length_z", 1.0), col("d", 1.0), col("u", 0.0), col("v", 0.0),
+            col("w", 0.0), col("p", 1.0))
+        cfg.kh_amp = kh_amp
+        self._h = _amr_lib().or_amr_create(C.byref(cfg))
+        if not self._h:
+            raise ValueError("bad adaptive-run configuration")
+
+    def step(self, n=1) -> int:
+        return _amr_lib().or_amr_step(self._h, n)
+
+    def state(self) -> dict:
+        s = OrAmrState()
+        _amr_lib().or_amr_get(self._h, C.byref(s))
+        ng, nc, nl = s.ngridmax, s.ncell, s.nlevelmax
+
+        def ints(p, n):
+            return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+        def dbl(p, n):
+            return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+        return dict(ncoarse=s.ncoarse, ngridmax=ng, ncell=nc, nlevelmax=nl, father=ints(s.father, ng),
+                    son=ints(s.son, nc), nbor=ints(s.nbor, 6 * ng), next=ints(s.next, ng),
+                    headl=ints(s.headl, nl + 1), numbl=ints(s.numbl, nl + 1), flag1=ints(s.flag1, nc),
+                    xg=dbl(s.xg, 3 * ng), uold=dbl(s.uold, 5 * nc), unew=dbl(s.unew, 5 * nc), t=s.t,
+                    nstep=s.nstep, dtnew=dbl(s.dtnew, 33), cell_updates=s.cell_updates, status=s.status)
+
+    def close(self):
+        if self._h:
+            _amr_lib().or_amr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
