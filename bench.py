@@ -19,6 +19,7 @@ import platform
 import subprocess
 import sys
 import time
+from ctypes import byref, c_double as C_double, c_longlong as C_longlong
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ramses-cpp_amd"))
@@ -126,6 +127,70 @@ def measure_ensemble(args, rhydro, np):
     }
 
 
+def measure_amr(args, rhydro, np, which):
+    """Adaptive runs with the state on the GPU (rh_amr_*): the AMR Sedov blast
+    (reference levels 7-10, err_grad_d = err_grad_p = 0.1) or the
+    Kelvin-Helmholtz slab (levels 8-11), timed over whole coarse steps
+    (hydro, refinement criteria, tree edits, prolongation, restriction), in
+    the FP64 form.  Cell-updates = leaf cells of every godunov_fine call.  The
+    first two coarse steps are also run through the oracle on one host core
+    and compared value for value."""
+    if which == "sedov":
+        kw = dict(levelmin=7, levelmax=10, ngridmax=300000, err_grad_d=0.1, err_grad_p=0.1,
+                  regions=rhydro.sedov_regions(7))
+        steps = args.amr_steps
+        name = ("AMR Sedov blast: 64^3 base (reference level 7) refined to level 10 on density and pressure "
+                "gradients (err_grad 0.1), subcycled, gamma=1.4, HLLC, slope_type 1, interpol_type 1")
+    else:
+        kw = dict(levelmin=8, levelmax=11, ngridmax=4500000, err_grad_d=0.2, ic="kh", kh_amp=0.01)
+        steps = args.kh_steps
+        name = ("AMR Kelvin-Helmholtz slab: 128^3 base (reference level 8) refined to level 11 on density "
+                "gradients (err_grad_d 0.2), subcycled, gamma=1.4, HLLC, slope_type 1")
+    common = dict(gamma=1.4, slope_type=1, riemann="hllc", interpol_type=1, interpol_var=0)
+    L = rhydro.lib()
+    tc0 = time.perf_counter()
+    run = rhydro.AmrRun(**kw, **common)
+    t_init = time.perf_counter() - tc0
+    ctx = run.solver_context()
+    L.rh_timers_enable(ctx, 1)
+    L.rh_timers_reset(ctx)
+    _, _, cu0 = run.stats()
+    t0 = time.perf_counter()
+    run.step(steps)
+    wall = time.perf_counter() - t0
+    t_sim, nstep, cu1 = run.stats()
+    ms, n = C_double(), C_longlong()
+    L.rh_timers_read(ctx, rhydro.TK_GODUNOV, byref(ms), byref(n))
+    st = run.state()
+    run.close()
+    out = {"workload": name, "coarse_steps": steps, "fine_steps": nstep, "cell_updates": int(cu1 - cu0),
+           "value": (cu1 - cu0) / wall, "wall_s": wall, "init_s": t_init, "godunov_ms_total": ms.value,
+           "godunov_launches": n.value, "octs_per_level": st["numbl"].tolist()}
+    print(f"[bench] amr {which}: {steps} coarse steps, {cu1 - cu0} cell-updates in {wall:.3f}s "
+          f"(godunov {ms.value:.1f} ms), lists {st['numbl'].tolist()}", file=sys.stderr, flush=True)
+    if not args.no_cpu and which == "sedov":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        okw = dict(kw, riemann=rhydro.resolve_riemann("hllc"), gamma=1.4, slope_type=1, interpol_type=1)
+        o = oracle.OracleAmr(**okw)
+        g = rhydro.AmrRun(**kw, **common)
+        ncmp = 2
+        tcpu0 = time.perf_counter()
+        o.step(ncmp)
+        tcpu = time.perf_counter() - tcpu0
+        g.step(ncmp)
+        so, sg = o.state(), g.state()
+        same = all(np.array_equal(so[k], sg[k]) for k in ("numbl", "son", "father", "nbor", "next", "uold", "unew"))
+        out.update({"cpu_value": so["cell_updates"] / tcpu, "cpu_kind": "port", "cpu_cores": 1,
+                    "cpu_sample": f"the first {ncmp} coarse steps through the oracle",
+                    "identical_to_oracle_after_2_steps": bool(same)})
+        g.close()
+        o.close()
+        print(f"[bench] amr {which}: oracle {ncmp} steps {tcpu:.1f}s identical={same}", file=sys.stderr, flush=True)
+    return out
+
+
 def measure_sedov64(args, rhydro, np):
     """The 64^3 Sedov blast (reference level 7, 262,144 cells; p = 1e-5
     outside, the unit energy in the 8 central cells), gamma = 1.4, HLLC,
@@ -186,6 +251,8 @@ def main() -> int:
     ap.add_argument("--boxes", type=int, default=96, help="boxes of the shock-tube ensemble (0: skip it)")
     ap.add_argument("--no-variants", action="store_true", help="skip the HLL / slope-2 variants of the workload")
     ap.add_argument("--no-sedov", action="store_true", help="skip the 64^3 Sedov run")
+    ap.add_argument("--amr-steps", type=int, default=10, help="coarse steps of the AMR Sedov run (0: skip)")
+    ap.add_argument("--kh-steps", type=int, default=1, help="coarse steps of the AMR Kelvin-Helmholtz run (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -307,6 +374,8 @@ def main() -> int:
 
     ens_info = measure_ensemble(args, rhydro, np) if args.boxes > 0 else None
     sedov_info = measure_sedov64(args, rhydro, np) if not args.no_sedov else None
+    amr_sedov = measure_amr(args, rhydro, np, "sedov") if args.amr_steps > 0 else None
+    amr_kh = measure_amr(args, rhydro, np, "kh") if args.kh_steps > 0 else None
 
     value = nleaf * args.steps / fp64["wall"]
     achieved = fp64["achieved"]
@@ -370,6 +439,8 @@ def main() -> int:
             "variants": variants,
             "ensemble": ens_info,
             "sedov64": sedov_info,
+            "amr_sedov": amr_sedov,
+            "amr_kelvin_helmholtz": amr_kh,
         },
         "roofline": {
             "bound": "hbm",

@@ -841,6 +841,10 @@ void* or_amr_create(const or_amr_config* cfg) {
                               cfg->length_x[i], cfg->length_y[i], cfg->length_z[i], cfg->d_region[i],
                               cfg->u_region[i], cfg->v_region[i], cfg->w_region[i], cfg->p_region[i]});
     a->initialize();
+    if (a->status != 0) {
+        delete a;
+        return nullptr;
+    }
     return a;
 }
 

@@ -1,0 +1,825 @@
+// rh_amr.cpp -- adaptive runs with the state resident on the GPU
+// (rhydro_amr.h).
+//
+// The host owns the tree (AmrGrid's integer arrays, ncpu = 1) and edits it as
+// the reference's tree updater does; every operation on the state runs on the
+// device: the hydro step through the context's kernels, the gradient
+// refinement criteria (k_grad_flags: one byte per cell comes back), the
+// prolongation into new octs (k_new_octs) and the restriction (k_restrict).
+// After a tree edit the context's mesh epoch moves, so its level tables are
+// rebuilt on next use, and the father/nbor arrays the AMR kernels read are
+// uploaded again.
+//
+// Equivalences relied on (ncpu = 1, nboundary = 0):
+//   * cpu_map is 1 everywhere and authorize_fine sets flag2 = 1 for every
+//     cell before make_grid_fine reads it, so neither array is kept;
+//   * the neighbours make_grid_fine reads (TreeUpdater.cpp:82-118) are
+//     get_nbor_cells / get_nbor_of_coarse of the refined cell, i.e.
+//     rh::nbor_cells_of_cell (periodic coarse wrap, no boundaries);
+//   * new octs of one make_grid_fine call read only cells of the refined
+//     list and write only their own children, so they are prolongated
+//     together in one launch after the tree edits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rh_context.h"
+#include "rh_kernels.h"
+#include "rh_tables.h"
+#include "rhydro.h"
+#include "rhydro_amr.h"
+#include "rhydro_driver.h"
+
+struct rh_amr {
+    rh_ctx* ctx = nullptr;
+    rh_params hp{};
+    rh_amr_params ap{};
+    // AmrGrid (AmrGrid.hpp:15-33).
+    int nx = 1, ny = 1, nz = 1, ncoarse = 1, ngridmax = 1, ncell = 1, nlevelmax = 1;
+    double boxlen = 1.0, gamma = 1.4;
+    std::vector<int> father, son, nbor, next, prev, flag1, headl, taill, numbl;
+    std::vector<double> xg, uold, unew;
+    int headf = 0, tailf = 0, numbf = 0;
+    // Simulation (Simulation.hpp).
+    int levelmin = 1;
+    std::vector<int> nsubcycle;
+    std::vector<double> dtnew, dtold;
+    double t = 0.0;
+    int nstep = 0;
+    long long cell_updates = 0;
+    std::vector<int> reg_type;
+    std::vector<double> reg[11];  // x,y,z centre, lengths x,y,z, d, u, v, w, p
+    // Device buffers of the AMR kernels.
+    int* d_father = nullptr;
+    int* d_nbor = nullptr;
+    int* d_cells = nullptr;
+    int ncells_cap = 0;
+    unsigned char* d_flags = nullptr;
+    bool topo_dirty = true;
+    long long epoch = 1;
+    rh_mesh mesh{};
+    std::string err;
+};
+
+using namespace rhi;
+
+namespace {
+
+int afail(rh_amr* a, int code, const std::string& msg) {
+    a->err = msg;
+    if (a->ctx) a->ctx->err = msg;
+    return code;
+}
+
+#define RH_AHIP(a, call)                                                         \
+    do {                                                                         \
+        hipError_t _e = (call);                                                  \
+        if (_e != hipSuccess) return afail(a, RH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define RH_ACHK(call)            \
+    do {                         \
+        int _s = (call);         \
+        if (_s != RH_OK) return _s; \
+    } while (0)
+
+int cell(const rh_amr* a, int igrid, int ic) { return a->ncoarse + (ic - 1) * a->ngridmax + igrid; }
+
+void refresh_mesh(rh_amr* a) {
+    rh_mesh& m = a->mesh;
+    m.ncoarse = a->ncoarse;
+    m.ngridmax = a->ngridmax;
+    m.ncell = a->ncell;
+    m.nvar = 5;
+    m.nx = a->nx;
+    m.ny = a->ny;
+    m.nz = a->nz;
+    m.nlevelmax = a->nlevelmax;
+    m.ncpu = 1;
+    m.nboundary = 0;
+    m.myid = 1;
+    m.father = a->father.data();
+    m.son = a->son.data();
+    m.nbor = a->nbor.data();
+    m.next = a->next.data();
+    m.headl = a->headl.data();
+    m.uold = a->uold.data();
+    m.unew = a->unew.data();
+    m.f = nullptr;
+    m.mesh_epoch = a->epoch;
+}
+
+// The tree changed: the context's level tables and the AMR kernels' copies
+// of father/nbor are stale.
+void tree_changed(rh_amr* a) {
+    a->epoch++;
+    a->ctx->mesh.mesh_epoch = a->epoch;
+    a->mesh.mesh_epoch = a->epoch;
+    a->topo_dirty = true;
+}
+
+int upload_topology(rh_amr* a) {
+    if (!a->topo_dirty) return RH_OK;
+    RH_AHIP(a, hipMemcpyAsync(a->d_father, a->father.data(), (size_t)a->ngridmax * sizeof(int),
+                              hipMemcpyHostToDevice, a->ctx->stream));
+    RH_AHIP(a, hipMemcpyAsync(a->d_nbor, a->nbor.data(), (size_t)6 * a->ngridmax * sizeof(int),
+                              hipMemcpyHostToDevice, a->ctx->stream));
+    a->topo_dirty = false;
+    return RH_OK;
+}
+
+// ---- AmrGrid lists (AmrGrid.cpp:13-70, 259-302) ---------------------------
+
+void allocate(rh_amr* a) {
+    a->ncoarse = a->nx * a->ny * a->nz;
+    a->ncell = a->ncoarse + 8 * a->ngridmax;
+    const int nl = a->nlevelmax + 1;
+    a->headl.assign(nl, 0);
+    a->taill.assign(nl, 0);
+    a->numbl.assign(nl, 0);
+    a->next.assign(a->ngridmax, 0);
+    a->prev.assign(a->ngridmax, 0);
+    a->father.assign(a->ngridmax, 0);
+    a->son.assign(a->ncell, 0);
+    a->nbor.assign((size_t)6 * a->ngridmax, 0);
+    a->flag1.assign(a->ncell, 0);
+    a->xg.assign((size_t)3 * a->ngridmax, 0.0);
+    a->uold.assign((size_t)5 * a->ncell, 0.0);
+    a->unew.assign((size_t)5 * a->ncell, 0.0);
+    a->headf = 1;
+    a->tailf = a->ngridmax;
+    for (int i = 1; i <= a->ngridmax; ++i) {
+        a->next[i - 1] = (i < a->ngridmax) ? i + 1 : 0;
+        a->prev[i - 1] = (i > 1) ? i - 1 : 0;
+    }
+    a->numbf = a->ngridmax;
+}
+
+int get_free_grid(rh_amr* a) {
+    if (a->headf == 0) return 0;
+    int ig = a->headf;
+    a->headf = a->next[ig - 1];
+    if (a->headf == 0) a->tailf = 0;
+    else a->prev[a->headf - 1] = 0;
+    a->next[ig - 1] = 0;
+    a->prev[ig - 1] = 0;
+    a->numbf--;
+    return ig;
+}
+
+void free_grid(rh_amr* a, int ig) {
+    for (int i = 0; i < 6; ++i) a->nbor[(size_t)i * a->ngridmax + ig - 1] = 0;
+    int fc = a->father[ig - 1];
+    if (fc > 0) a->son[fc - 1] = 0;
+    a->father[ig - 1] = 0;
+    for (int ic = 1; ic <= 8; ++ic) {
+        int idc = cell(a, ig, ic) - 1;
+        a->son[idc] = 0;
+        a->flag1[idc] = 0;
+    }
+    if (a->tailf == 0) {
+        a->headf = a->tailf = ig;
+        a->prev[ig - 1] = a->next[ig - 1] = 0;
+    } else {
+        a->next[a->tailf - 1] = ig;
+        a->prev[ig - 1] = a->tailf;
+        a->next[ig - 1] = 0;
+        a->tailf = ig;
+    }
+    a->numbf++;
+}
+
+void add_to_list(rh_amr* a, int ig, int il) {
+    if (a->headl[il] == 0) {
+        a->headl[il] = a->taill[il] = ig;
+        a->prev[ig - 1] = a->next[ig - 1] = 0;
+    } else {
+        int tail = a->taill[il];
+        a->next[tail - 1] = ig;
+        a->prev[ig - 1] = tail;
+        a->next[ig - 1] = 0;
+        a->taill[il] = ig;
+    }
+    a->numbl[il]++;
+}
+
+int count(const rh_amr* a, int il) { return (il >= 0 && il <= a->nlevelmax) ? a->numbl[il] : 0; }
+
+// ---- geometry (AmrGrid.hpp:69-80, AmrGrid.cpp:331-387, 474-511) -----------
+
+int depth_of(const rh_amr* a, int ig) {
+    int level = 1, cur = ig;
+    while (cur > 0) {
+        int fc = a->father[cur - 1];
+        if (fc <= a->ncoarse) break;
+        cur = ((fc - a->ncoarse - 1) % a->ngridmax) + 1;
+        level++;
+    }
+    return level;
+}
+
+int cell_level(const rh_amr* a, int icell) {
+    if (icell <= a->ncoarse) return 0;
+    return depth_of(a, ((icell - a->ncoarse - 1) % a->ngridmax) + 1);
+}
+
+void cell_center(const rh_amr* a, int icell, double xc[3]) {
+    xc[0] = xc[1] = xc[2] = 0.5 * a->boxlen;
+    if (icell <= a->ncoarse) {
+        int idx = icell - 1;
+        int iz = idx / (a->nx * a->ny);
+        idx %= (a->nx * a->ny);
+        int iy = idx / a->nx, ix = idx % a->nx;
+        double dxc = a->boxlen / std::max({a->nx, a->ny, a->nz});
+        xc[0] = (ix + 0.5) * dxc;
+        xc[1] = (iy + 0.5) * dxc;
+        xc[2] = (iz + 0.5) * dxc;
+        return;
+    }
+    int ig = ((icell - a->ncoarse - 1) % a->ngridmax) + 1;
+    int ic0 = (icell - a->ncoarse - 1) / a->ngridmax;
+    int ixyz[3] = {ic0 & 1, (ic0 & 2) >> 1, (ic0 & 4) >> 2};
+    double dx = a->boxlen / (double)(a->nx * (1 << depth_of(a, ig)));
+    double scale = a->boxlen / (double)a->nx;
+    for (int d = 0; d < 3; ++d) xc[d] = a->xg[(size_t)d * a->ngridmax + ig - 1] * scale + (double)(ixyz[d] - 0.5) * dx;
+}
+
+int find_cell(const rh_amr* a, const double x[3], int level) {
+    double dxc = a->boxlen / std::max({a->nx, a->ny, a->nz});
+    int ix = std::clamp((int)(x[0] / dxc), 0, a->nx - 1);
+    int iy = std::clamp((int)(x[1] / dxc), 0, a->ny - 1);
+    int iz = std::clamp((int)(x[2] / dxc), 0, a->nz - 1);
+    int cur = iz * a->nx * a->ny + iy * a->nx + ix + 1;
+    int lev = 1;
+    const double scale = a->boxlen / (double)a->nx;
+    while (lev <= level && a->son[cur - 1] > 0) {
+        int ig = a->son[cur - 1];
+        lev++;
+        int ixc = (x[0] > a->xg[(size_t)0 * a->ngridmax + ig - 1] * scale) ? 1 : 0;
+        int iyc = (x[1] > a->xg[(size_t)1 * a->ngridmax + ig - 1] * scale) ? 1 : 0;
+        int izc = (x[2] > a->xg[(size_t)2 * a->ngridmax + ig - 1] * scale) ? 1 : 0;
+        cur = cell(a, ig, izc * 4 + iyc * 2 + ixc + 1);
+    }
+    return cur;
+}
+
+// Every 27-neighbour of a cell exists at its level and is refined.
+bool neighbours_refined(const rh_amr* a, int icell) {
+    int level = cell_level(a, icell);
+    double xc[3];
+    cell_center(a, icell, xc);
+    double dxl = a->boxlen / (double)(a->nx * (1 << level));
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                double x[3] = {xc[0] + dx * dxl, xc[1] + dy * dxl, xc[2] + dz * dxl};
+                for (int d = 0; d < 3; ++d) {
+                    if (x[d] < 0.0) x[d] += a->boxlen;
+                    else if (x[d] >= a->boxlen) x[d] -= a->boxlen;
+                }
+                int c = find_cell(a, x, level);
+                if (cell_level(a, c) != level) return false;
+                if (a->son[c - 1] == 0) return false;
+            }
+    return true;
+}
+
+// get_nbor_cells_exact (AmrGrid.cpp:449-472): same-level neighbour cells of
+// cell ic (1-based) of grid ig, 0 where the neighbour oct does not exist.
+void nbor_exact(const rh_amr* a, int ig, int ic, int icn[6]) {
+    const int ic0 = ic - 1;
+    for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+            const int par = (ic0 >> d) & 1;
+            const int icS = (ic0 ^ (1 << d)) + 1;
+            int v = 0;
+            if (par != side) {
+                v = cell(a, ig, icS);
+            } else {
+                int nb = a->nbor[(size_t)(2 * d + side) * a->ngridmax + ig - 1];
+                if (nb > a->ncell) nb = 0;
+                if (nb > 0) {
+                    int child = a->son[nb - 1];
+                    if (child > 0) v = cell(a, child, icS);
+                }
+            }
+            icn[2 * d + side] = v;
+        }
+}
+
+// ---- tree updater (TreeUpdater.cpp) ---------------------------------------
+
+bool child_active(const rh_amr* a, int idc0) {
+    int child = a->son[idc0];
+    if (child <= 0) return false;
+    for (int ics = 1; ics <= 8; ++ics) {
+        int s = cell(a, child, ics) - 1;
+        if (a->son[s] > 0 || a->flag1[s] == 1) return true;
+    }
+    return false;
+}
+
+void smooth_fine(rh_amr* a, int il) {
+    std::vector<int> ok;
+    if (il == 1) {
+        ok.assign(a->ncoarse, 0);
+        for (int ic = 1; ic <= a->ncoarse; ++ic) {
+            if (a->flag1[ic - 1] != 0) continue;
+            int icn[6];
+            rh::nbor_cells_of_cell(a->mesh, ic, icn);
+            int nf = 0;
+            for (int k = 0; k < 6; ++k)
+                if (icn[k] > 0 && icn[k] <= a->ncoarse && a->flag1[icn[k] - 1] == 1) nf++;
+            if (nf > 0) ok[ic - 1] = 1;
+        }
+        for (int i = 0; i < a->ncoarse; ++i)
+            if (ok[i]) a->flag1[i] = 1;
+        return;
+    }
+    std::vector<int> set;
+    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1])
+        for (int ic = 1; ic <= 8; ++ic) {
+            int idc = cell(a, ig, ic) - 1;
+            if (a->flag1[idc] != 0) continue;
+            int icn[6];
+            nbor_exact(a, ig, ic, icn);
+            int nf = 0;
+            for (int k = 0; k < 6; ++k)
+                if (icn[k] > 0 && icn[k] <= a->ncell && a->flag1[icn[k] - 1] == 1) nf++;
+            if (nf > 0) set.push_back(idc);
+        }
+    for (int idc : set) a->flag1[idc] = 1;
+}
+
+void test_flag(rh_amr* a, int il) {
+    if (il == 1) {
+        for (int ic = 1; ic <= a->ncoarse; ++ic)
+            if (child_active(a, ic - 1)) a->flag1[ic - 1] = 1;
+        return;
+    }
+    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1])
+        for (int ic = 1; ic <= 8; ++ic) {
+            int idc = cell(a, ig, ic) - 1;
+            if (child_active(a, idc)) a->flag1[idc] = 1;
+        }
+}
+
+void ensure_ref_rules(rh_amr* a, int il) {
+    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1]) {
+        int fc = a->father[ig - 1];
+        bool ok = fc > 0 && neighbours_refined(a, fc);
+        if (!ok)
+            for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
+    }
+}
+
+// Gradient criteria of the coarse cells (TreeUpdater.cpp:459-480), on the
+// host from a copy of their state (the coarse level is not a level list).
+struct CState {
+    double d, v[3], p;
+};
+
+CState cstate(const rh_amr* a, const std::vector<double>& u, int id) {
+    CState s{0.0, {0.0, 0.0, 0.0}, 0.0};
+    if (id <= 0 || id > a->ncoarse) return s;
+    auto U = [&](int iv) { return u[(size_t)(iv - 1) * a->ncoarse + id - 1]; };
+    s.d = std::max(U(1), 1e-10);
+    double v2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        s.v[k] = U(2 + k) / s.d;
+        v2 += s.v[k] * s.v[k];
+    }
+    double ek = 0.5 * s.d * v2;
+    double em = 0.0, e_non = 0.0;
+    s.p = std::max((U(5) - ek - em - e_non) * (a->gamma - 1.0), s.d * 1e-10);
+    return s;
+}
+
+double egrad(double l, double c, double r, double fl) {
+    return 2.0 * std::max(std::abs(r - c) / (r + c + fl), std::abs(c - l) / (c + l + fl));
+}
+
+int coarse_gradients(rh_amr* a) {
+    std::vector<double> u((size_t)5 * a->ncoarse);
+    for (int iv = 0; iv < 5; ++iv)
+        RH_AHIP(a, hipMemcpyAsync(u.data() + (size_t)iv * a->ncoarse, a->ctx->d_uold + (size_t)iv * a->ncell,
+                                  (size_t)a->ncoarse * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
+    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
+    const double ed = a->ap.err_grad_d, ep = a->ap.err_grad_p, ev = a->ap.err_grad_v;
+    for (int i = 1; i <= a->ncoarse; ++i) {
+        if (a->flag1[i - 1] != 0) continue;
+        CState sc = cstate(a, u, i);
+        bool ok = false;
+        for (int d = 0; d < 3 && !ok; ++d) {
+            int nxd = d == 0 ? a->nx : (d == 1 ? a->ny : a->nz);
+            int nskip = d == 0 ? 1 : (d == 1 ? a->nx : a->nx * a->ny);
+            int il = i - nskip;
+            if (il < 1) il += nxd * nskip;
+            int ir = i + nskip;
+            if (ir > a->ncoarse) ir -= nxd * nskip;
+            CState sl = cstate(a, u, il), sr = cstate(a, u, ir);
+            if (ed > 0.0 && egrad(sl.d, sc.d, sr.d, 1e-10) > ed) ok = true;
+            else if (ep > 0.0 && egrad(sl.p, sc.p, sr.p, 1e-10) > ep) ok = true;
+            else if (ev > 0.0) {
+                const double fl = 1e-10, g = a->gamma;
+                double cg = std::sqrt(std::max(g * sl.p / sl.d, fl * fl));
+                double cm = std::sqrt(std::max(g * sc.p / sc.d, fl * fl));
+                double cd = std::sqrt(std::max(g * sr.p / sr.d, fl * fl));
+                double e = 2.0 * std::max(std::abs(sr.v[d] - sc.v[d]) /
+                                              (cd + cm + std::abs(sr.v[d]) + std::abs(sc.v[d]) + fl),
+                                          std::abs(sc.v[d] - sl.v[d]) /
+                                              (cm + cg + std::abs(sc.v[d]) + std::abs(sl.v[d]) + fl));
+                if (e > ev) ok = true;
+            }
+        }
+        if (ok) a->flag1[i - 1] = 1;
+    }
+    return RH_OK;
+}
+
+// Gradient criteria of a level list on the device; flags come back per cell.
+int list_gradients(rh_amr* a, int il) {
+    const rh::LevelDev* lv;
+    int st = get_level(a->ctx, il, &lv);
+    if (st) return st;
+    if (lv->noct == 0) return RH_OK;
+    RH_ACHK(upload_topology(a));
+    RH_AHIP(a, rh::launch_grad_flags(mesh_dev(a->ctx), *lv, a->d_nbor, a->ap.err_grad_d, a->ap.err_grad_p,
+                                     a->ap.err_grad_v, a->gamma, a->d_flags, a->ctx->stream));
+    std::vector<unsigned char> fl((size_t)lv->noct * 8);
+    RH_AHIP(a, hipMemcpyAsync(fl.data(), a->d_flags, fl.size(), hipMemcpyDeviceToHost, a->ctx->stream));
+    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
+    int pos = 0;
+    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1], ++pos)
+        for (int ic = 1; ic <= 8; ++ic) {
+            int idc = cell(a, ig, ic) - 1;
+            if (a->flag1[idc] == 0 && fl[(size_t)pos * 8 + ic - 1]) a->flag1[idc] = 1;
+        }
+    return RH_OK;
+}
+
+// TreeUpdater.cpp:328-583
+int flag_fine(rh_amr* a, int il, int icount, int nsub) {
+    if (il >= a->nlevelmax) return RH_OK;
+    const int lmin = a->levelmin;
+    if (il == 1) {
+        for (int i = 0; i < a->ncoarse; ++i) a->flag1[i] = 0;
+        for (int i = 1; i <= a->ncoarse; ++i)
+            if (il < lmin || child_active(a, i - 1)) a->flag1[i - 1] = 1;
+    } else {
+        for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1])
+            for (int ic = 1; ic <= 8; ++ic) {
+                int idc = cell(a, ig, ic) - 1;
+                a->flag1[idc] = (il < lmin || child_active(a, idc)) ? 1 : 0;
+            }
+    }
+    if (il >= lmin) smooth_fine(a, il);
+    if (il >= lmin && (a->ap.err_grad_d > 0.0 || a->ap.err_grad_p > 0.0 || a->ap.err_grad_v > 0.0))
+        RH_ACHK(il == 1 ? coarse_gradients(a) : list_gradients(a, il));
+    if (il > lmin && icount < nsub) test_flag(a, il);
+    for (int i = 0; i < a->ap.nexpand; ++i) smooth_fine(a, il);
+    if (il > lmin && icount < nsub) ensure_ref_rules(a, il);
+    return RH_OK;
+}
+
+// TreeUpdater.cpp:27-150: topology on the host, prolongation on the device.
+int make_grid_fine(rh_amr* a, int il) {
+    if (il >= a->nlevelmax) return RH_OK;
+    std::vector<int> cells;
+    if (il == 1) {
+        for (int ic = 1; ic <= a->ncoarse; ++ic)
+            if (a->flag1[ic - 1] == 1 && a->son[ic - 1] == 0) cells.push_back(ic);
+    } else {
+        for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1])
+            for (int ic = 1; ic <= 8; ++ic) {
+                int idc = cell(a, ig, ic);
+                if (a->flag1[idc - 1] == 1 && a->son[idc - 1] == 0) cells.push_back(idc);
+            }
+    }
+    if (cells.empty()) return RH_OK;
+    std::vector<int> idx;
+    idx.reserve(cells.size() * 8);
+    for (int ic_coarse : cells) {
+        int new_ig = get_free_grid(a);
+        if (new_ig == 0) return afail(a, RH_ERR_ARG, "ngridmax exhausted (the run does not resize)");
+        a->son[ic_coarse - 1] = new_ig;
+        a->father[new_ig - 1] = ic_coarse;
+        int icn[6];
+        rh::nbor_cells_of_cell(a->mesh, ic_coarse, icn);
+        for (int f = 0; f < 6; ++f) a->nbor[(size_t)f * a->ngridmax + new_ig - 1] = icn[f];
+        if (il == 1) {
+            int idx0 = ic_coarse - 1;
+            int x[3] = {idx0 % a->nx, (idx0 % (a->nx * a->ny)) / a->nx, idx0 / (a->nx * a->ny)};
+            for (int d = 0; d < 3; ++d) a->xg[(size_t)d * a->ngridmax + new_ig - 1] = (double)x[d] + 0.5;
+        } else {
+            int ig = ((ic_coarse - 1 - a->ncoarse) % a->ngridmax) + 1;
+            int ic0 = (ic_coarse - 1 - a->ncoarse) / a->ngridmax;
+            int ixyz[3] = {ic0 & 1, (ic0 & 2) >> 1, (ic0 & 4) >> 2};
+            for (int d = 0; d < 3; ++d) {
+                double dxl = 1.0 / (double)(1 << (il - 1));
+                a->xg[(size_t)d * a->ngridmax + new_ig - 1] =
+                    a->xg[(size_t)d * a->ngridmax + ig - 1] + (double)(ixyz[d] - 0.5) * dxl;
+            }
+        }
+        idx.push_back(ic_coarse);
+        for (int f = 0; f < 6; ++f) idx.push_back(icn[f]);
+        idx.push_back(new_ig);
+        add_to_list(a, new_ig, il + 1);
+    }
+    const int n = (int)cells.size();
+    if (n > a->ncells_cap) {
+        if (a->d_cells) (void)hipFree(a->d_cells);
+        a->d_cells = nullptr;
+        a->ncells_cap = 0;
+        RH_AHIP(a, hipMalloc(&a->d_cells, (size_t)n * 8 * sizeof(int)));
+        a->ncells_cap = n;
+    }
+    RH_AHIP(a, hipMemcpyAsync(a->d_cells, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice,
+                              a->ctx->stream));
+    RH_AHIP(a, rh::launch_new_octs(mesh_dev(a->ctx), n, a->d_cells, gas(a->ctx, a->gamma), a->hp.interpol_var,
+                                   a->ctx->stream));
+    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));  // idx goes out of scope
+    tree_changed(a);
+    return RH_OK;
+}
+
+// TreeUpdater.cpp:152-193
+void remove_grid_fine(rh_amr* a, int il) {
+    if (il >= a->nlevelmax) return;
+    bool changed = false;
+    for (int ig = a->headl[il]; ig > 0;) {
+        int nxt = a->next[ig - 1];
+        int fc = a->father[ig - 1];
+        bool remove = false;
+        if (fc > 0 && a->flag1[fc - 1] == 0) {
+            remove = true;
+            for (int ic = 1; ic <= 8; ++ic)
+                if (a->son[cell(a, ig, ic) - 1] > 0) {
+                    remove = false;
+                    break;
+                }
+        }
+        if (remove) {
+            for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
+            int p = a->prev[ig - 1], n = a->next[ig - 1];
+            if (p > 0) a->next[p - 1] = n;
+            else a->headl[il] = n;
+            if (n > 0) a->prev[n - 1] = p;
+            else a->taill[il] = p;
+            a->numbl[il]--;
+            free_grid(a, ig);
+            changed = true;
+        }
+        ig = nxt;
+    }
+    if (changed) tree_changed(a);
+}
+
+// TreeUpdater.cpp:195-215 on the device.
+int restrict_fine(rh_amr* a, int il) {
+    if (il + 1 > a->nlevelmax || count(a, il + 1) == 0) return RH_OK;
+    const rh::LevelDev* lv;
+    RH_ACHK(get_level(a->ctx, il + 1, &lv));
+    RH_ACHK(upload_topology(a));
+    RH_AHIP(a, rh::launch_restrict(mesh_dev(a->ctx), *lv, a->d_father, a->ctx->stream));
+    return RH_OK;
+}
+
+// Initializer.cpp:19-157 (apply_all) computed on the host for every coarse
+// and listed cell, then uploaded.  Cells that no region covers keep their
+// state, so the current device state is the starting point.
+int apply_all(rh_amr* a) {
+    const bool kh = a->ap.ic_kind == 1;
+    auto set = [&](int idc, double d, double u, double v, double w, double p) {
+        auto U = [&](int iv) -> double& { return a->uold[(size_t)(iv - 1) * a->ncell + idc - 1]; };
+        U(1) = d;
+        U(2) = d * u;
+        U(3) = d * v;
+        U(4) = d * w;
+        double v2 = u * u;
+        v2 += v * v;
+        v2 += w * w;
+        double e_kin = 0.5 * d * v2;
+        double e_int = p / (a->gamma - 1.0);
+        double e_nonthermal = 0.0;
+        U(5) = e_kin + e_int + e_nonthermal;
+    };
+    auto apply = [&](int idc) {
+        double xc[3];
+        cell_center(a, idc, xc);
+        const double x = xc[0], y = xc[1], z = xc[2];
+        if (kh) {
+            const double pi = 3.14159265358979323846;
+            const bool in = std::abs(y - 0.5) < 0.25;
+            set(idc, in ? 2.0 : 1.0, in ? 0.5 : -0.5, a->ap.kh_amp * std::sin(4.0 * pi * x), 0.0, 2.5);
+            return;
+        }
+        for (size_t r = 0; r < a->reg_type.size(); ++r) {
+            const double xr = a->reg[0][r], yr = a->reg[1][r], zr = a->reg[2][r];
+            bool in;
+            if (a->reg_type[r] == 0)
+                in = std::abs(x - xr) < 0.5 * a->reg[3][r] && std::abs(y - yr) < 0.5 * a->reg[4][r] &&
+                     std::abs(z - zr) < 0.5 * a->reg[5][r];
+            else
+                in = std::pow(x - xr, 2) + std::pow(y - yr, 2) + std::pow(z - zr, 2) < std::pow(a->reg[3][r], 2);
+            if (in) set(idc, a->reg[6][r], a->reg[7][r], a->reg[8][r], a->reg[9][r], a->reg[10][r]);
+        }
+    };
+    RH_AHIP(a, hipMemcpyAsync(a->uold.data(), a->ctx->d_uold, a->uold.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, a->ctx->stream));
+    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
+    for (int i = 1; i <= a->ncoarse; ++i) apply(i);
+    for (int il = 1; il <= a->nlevelmax; ++il)
+        for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1])
+            for (int ic = 1; ic <= 8; ++ic) apply(cell(a, ig, ic));
+    RH_AHIP(a, hipMemcpyAsync(a->ctx->d_uold, a->uold.data(), a->uold.size() * sizeof(double),
+                              hipMemcpyHostToDevice, a->ctx->stream));
+    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
+    return RH_OK;
+}
+
+int legacy_init_flow(rh_amr* a) {
+    RH_ACHK(apply_all(a));
+    for (int il = a->nlevelmax - 1; il >= 0; --il) RH_ACHK(restrict_fine(a, il));
+    return RH_OK;
+}
+
+// Simulation.cpp:160-255 (refinement part of initialize).
+int initialize(rh_amr* a) {
+    RH_ACHK(apply_all(a));
+    a->nsubcycle.assign(33, 1);
+    for (int i = a->levelmin; i < 33; ++i) a->nsubcycle[i] = 2;
+    a->dtnew.assign(33, 0.0);
+    a->dtold.assign(33, 0.0);
+    const int lmin = a->levelmin, lmax = a->nlevelmax;
+    for (int il_ref = 1; il_ref <= lmin; ++il_ref)
+        for (int il = 1; il <= lmax; ++il) {
+            RH_ACHK(flag_fine(a, il, 1, 2));
+            RH_ACHK(make_grid_fine(a, il));
+        }
+    for (int il_ref = lmin + 1; il_ref <= lmax; ++il_ref) {
+        RH_ACHK(legacy_init_flow(a));
+        for (int il = 1; il <= lmax; ++il) {
+            RH_ACHK(flag_fine(a, il, 1, 2));
+            RH_ACHK(make_grid_fine(a, il));
+        }
+        if (count(a, il_ref) == 0) break;
+    }
+    a->nstep = 0;
+    return legacy_init_flow(a);
+}
+
+// Simulation.cpp:522-635
+int amr_step(rh_amr* a, int il, int icount) {
+    if (il > a->nlevelmax) return RH_OK;
+    if (count(a, il) == 0 && il > 0) return RH_OK;
+    if (a->levelmin < a->nlevelmax && (il == a->levelmin || icount > 1)) {
+        for (int i = il; i <= a->nlevelmax; ++i) RH_ACHK(make_grid_fine(a, i + 1));
+        for (int i = a->nlevelmax; i >= il; --i) remove_grid_fine(a, i + 1);
+    }
+    const double dx = a->boxlen / (double)(a->nx * (1 << il));
+    a->dtold[il] = a->dtnew[il];
+    double dtc = 0.0;
+    RH_ACHK(rh_courant(a->ctx, il, dx, a->gamma, a->ap.courant_factor, &dtc));
+    a->dtnew[il] = dtc;
+    if (il > a->levelmin) a->dtnew[il] = std::min(a->dtnew[il], a->dtnew[il - 1] / (double)a->nsubcycle[il - 1]);
+    const double dt = a->dtnew[il];
+    RH_ACHK(rh_set_unew(a->ctx, il));
+    const int nsub = il < (int)a->nsubcycle.size() ? a->nsubcycle[il] : 1;
+    if (il < a->nlevelmax) {
+        if (count(a, il + 1) > 0) {
+            for (int i = 1; i <= nsub; ++i) RH_ACHK(amr_step(a, il + 1, i));
+        } else {
+            a->dtold[il + 1] = a->dtnew[il] / (double)nsub;
+            a->dtnew[il + 1] = a->dtnew[il] / (double)nsub;
+            a->t += dt;
+            a->nstep++;
+        }
+    } else {
+        a->t += dt;
+        a->nstep++;
+    }
+    long long nleaf = 0;
+    RH_ACHK(rh_level_leaf_cells(a->ctx, il, &nleaf));
+    a->cell_updates += nleaf;
+    RH_ACHK(rh_godunov_fine(a->ctx, il, dt, dx));
+    RH_ACHK(rh_set_uold(a->ctx, il));
+    if (il < a->nlevelmax) RH_ACHK(restrict_fine(a, il));
+    return flag_fine(a, il + 1, icount, il > 0 ? a->nsubcycle[il - 1] : 1);
+}
+
+int step_coarse(rh_amr* a) {
+    if (a->levelmin < a->nlevelmax)
+        for (int il = 1; il <= a->levelmin - 1; ++il) RH_ACHK(make_grid_fine(a, il));
+    RH_ACHK(amr_step(a, a->levelmin, 1));
+    for (int il = a->levelmin; il >= 1; --il) RH_ACHK(flag_fine(a, il, 2, 2));
+    if (a->levelmin < a->nlevelmax)
+        for (int il = a->levelmin - 1; il >= 0; --il) RH_ACHK(restrict_fine(a, il));
+    return RH_OK;
+}
+
+void destroy(rh_amr* a) {
+    if (a->ctx) {
+        (void)hipSetDevice(a->hp.device);
+        for (void* p : {(void*)a->d_father, (void*)a->d_nbor, (void*)a->d_cells, (void*)a->d_flags})
+            if (p) (void)hipFree(p);
+        rh_destroy(a->ctx);
+    }
+    delete a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rh_amr_create(const rh_params* hydro, const rh_amr_params* ap, rh_amr** out) {
+    if (!hydro || !ap || !out) return RH_ERR_ARG;
+    *out = nullptr;
+    if (ap->levelmin < 1 || ap->levelmax < ap->levelmin || ap->levelmax > 30 || ap->ngridmax < 1 || ap->nx < 1 ||
+        ap->ny < 1 || ap->nz < 1 || ap->nexpand < 0 || (ap->ic_kind == 0 && ap->nreg > 0 && !ap->reg_type))
+        return RH_ERR_ARG;
+    const long long ncell = (long long)ap->nx * ap->ny * ap->nz + 8LL * ap->ngridmax;
+    if (ncell > INT_MAX / 5) return RH_ERR_ARG;
+    rh_amr* a = new rh_amr();
+    a->hp = *hydro;
+    a->ap = *ap;
+    a->nx = ap->nx;
+    a->ny = ap->ny;
+    a->nz = ap->nz;
+    a->ngridmax = ap->ngridmax;
+    a->nlevelmax = ap->levelmax;
+    a->levelmin = ap->levelmin;
+    a->boxlen = hydro->boxlen;
+    a->gamma = hydro->gamma;
+    if (ap->ic_kind == 0)
+        for (int r = 0; r < ap->nreg; ++r) {
+            a->reg_type.push_back(ap->reg_type[r]);
+            const double* cols[11] = {ap->x_center, ap->y_center, ap->z_center, ap->length_x, ap->length_y,
+                                      ap->length_z, ap->d_region, ap->u_region, ap->v_region, ap->w_region,
+                                      ap->p_region};
+            for (int k = 0; k < 11; ++k) a->reg[k].push_back(cols[k][r]);
+        }
+    a->ap.reg_type = nullptr;  // the run keeps its own copy
+    int st = rh_create(hydro, &a->ctx);
+    if (st != RH_OK) {
+        delete a;
+        return st;
+    }
+    allocate(a);
+    refresh_mesh(a);
+    st = rh_bind(a->ctx, &a->mesh, RH_DEVICE_RESIDENT);
+    if (st == RH_OK) {
+        hipError_t e = hipMalloc(&a->d_father, (size_t)a->ngridmax * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc(&a->d_nbor, (size_t)6 * a->ngridmax * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc(&a->d_flags, (size_t)8 * a->ngridmax);
+        if (e != hipSuccess) st = afail(a, RH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    if (st == RH_OK) st = initialize(a);
+    if (st != RH_OK) {
+        g_create_error = a->err.empty() ? rh_last_error(a->ctx) : a->err;
+        destroy(a);
+        return st;
+    }
+    *out = a;
+    return RH_OK;
+}
+
+int rh_amr_destroy(rh_amr* a) {
+    if (a) destroy(a);
+    return RH_OK;
+}
+
+int rh_amr_step(rh_amr* a, int nsteps) {
+    if (!a || nsteps < 0) return RH_ERR_ARG;
+    RH_AHIP(a, hipSetDevice(a->hp.device));
+    for (int k = 0; k < nsteps; ++k) RH_ACHK(step_coarse(a));
+    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
+    return RH_OK;
+}
+
+int rh_amr_mesh(rh_amr* a, rh_mesh* mesh, const int** numbl, const double** xg) {
+    if (!a) return RH_ERR_ARG;
+    RH_ACHK(rh_sync(a->ctx, RH_SYNC_TO_HOST, RH_UOLD | RH_UNEW));
+    refresh_mesh(a);
+    if (mesh) *mesh = a->mesh;
+    if (numbl) *numbl = a->numbl.data();
+    if (xg) *xg = a->xg.data();
+    return RH_OK;
+}
+
+int rh_amr_stats(rh_amr* a, double* t, int* nstep, long long* cell_updates) {
+    if (!a) return RH_ERR_ARG;
+    if (t) *t = a->t;
+    if (nstep) *nstep = a->nstep;
+    if (cell_updates) *cell_updates = a->cell_updates;
+    return RH_OK;
+}
+
+rh_ctx* rh_amr_context(rh_amr* a) { return a ? a->ctx : nullptr; }
+
+}  // extern "C"

@@ -89,6 +89,8 @@ void free_level(DevLevel& L);
 rh::MeshDev mesh_dev(const rh_ctx* c);
 rh::GasParams gas(const rh_ctx* c, double gamma);
 int ensure_dt(rh_ctx* c, int n);
+// Level tables of ilevel for the bound mesh, rebuilt when the epoch moved.
+int get_level(rh_ctx* c, int ilevel, const rh::LevelDev** out);
 void collect_timers(rh_ctx* c);
 // Frees every device buffer of the current binding (single or ensemble).
 void unbind(rh_ctx* c);

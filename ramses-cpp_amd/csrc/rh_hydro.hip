@@ -299,27 +299,28 @@ RH_DEV void solve_face(const TC ql[5], const TC qr[5], double F[5], const Gas<TC
 constexpr int G_OCTS = 16;
 constexpr int G_BLOCK = G_OCTS * 8;
 
-// Ordered accumulation of one leaf cell (HydroSolver.cpp:476-547 in
-// processing order).  s_r/s_k hold, for this thread, the six incoming
-// contributions and their keys (negative key: none).
-template <int NT>
-RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const double fs[5], double dtdx,
-                          const GasParams& gp, double (*s_r)[NT], long long (*s_k)[NT], int t) {
-    const long long KOWN = own_key * 8 + 6;
-    // ---- ordered accumulation (HydroSolver.cpp:476-547 in processing order)
-    // Sort the six incoming contributions by key (invalid ones last), then
-    // subtract those ordered before this cell, apply the cell's own update
-    // and clamps, and subtract the rest.
-    const long long KMAX = 0x7fffffffffffffffLL;
-    long long ks[6];
-    int ix[6];
+// A leaf cell's own update and clamps, HydroSolver.cpp:489-547.
+RH_DEV void own_update(double u[5], const double fs[5], double dtdx, const GasParams& gp) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        const long long kf = s_k[f][t];
-        ks[f] = kf < 0 ? KMAX : kf;
-        ix[f] = f;
+    for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] + dtdx * fs[iv];
+    double d_curr = sclamp(u[0], 1e-10, 1e6);
+    u[0] = d_curr;
+    for (int i = 1; i <= 3; ++i) u[i] = sclamp(u[i], -d_curr * 10.0, d_curr * 10.0);
+    double v2_curr = 0.0;
+    for (int i = 1; i <= 3; ++i) {
+        double v = u[i] / d_curr;
+        v2_curr = v2_curr + v * v;
     }
-    // Sorting network for 6 elements (12 compare-exchanges).
+    double ei_curr = u[4] - 0.5 * d_curr * v2_curr;
+    if (ei_curr < d_curr * 1e-10 / gp.gm1) {
+        double e_non = 0;
+        u[4] = d_curr * 1e-10 / gp.gm1 + 0.5 * d_curr * v2_curr + e_non;
+    }
+    u[4] = smin(u[4], d_curr * 1e8);
+}
+
+// Sort network for 6 keys (12 compare-exchanges) carrying slot indices.
+RH_DEV void sort6(long long ks[6], int ix[6]) {
 #define RH_CX(a, b)                                   \
     {                                                 \
         const bool sw = ks[b] < ks[a];                \
@@ -335,13 +336,81 @@ RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const dou
     RH_CX(1, 2) RH_CX(4, 5) RH_CX(0, 2) RH_CX(3, 5) RH_CX(0, 1) RH_CX(3, 4)
     RH_CX(1, 4) RH_CX(0, 3) RH_CX(2, 5) RH_CX(1, 3) RH_CX(2, 4) RH_CX(2, 3)
 #undef RH_CX
+}
+
+// Processing order of a cell's six incoming contributions, known from the
+// topology alone: keys[f] is the key of the contribution arriving through
+// face f (KMAX: none).  Returns the rank of each face's contribution packed
+// 3 bits per face, and the number of contributions ordered before the cell
+// itself (nbefore) and in total (nvalid).
+constexpr long long KMAX = 0x7fffffffffffffffLL;
+RH_DEV unsigned rank_contributions(const long long keys[6], long long own_key, int& nbefore, int& nvalid) {
+    long long ks[6];
+    int ix[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        ks[f] = keys[f];
+        ix[f] = f;
+    }
+    sort6(ks, ix);
+    const long long KOWN = own_key * 8 + 6;
+    unsigned pack = 0;
+    nbefore = 0;
+    nvalid = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        pack |= (unsigned)k << (3 * ix[k]);
+        nbefore += ks[k] < KOWN ? 1 : 0;
+        nvalid += ks[k] != KMAX ? 1 : 0;
+    }
+    return pack;
+}
+
+// Ordered accumulation with the contributions already stored in processing
+// order (s_r rows 5k..5k+4 hold the k-th one): the first nbefore before the
+// cell's own update and clamps, the rest after (HydroSolver.cpp:476-547).
+template <int NT>
+RH_DEV void ordered_apply_ranked(const MeshDev& m, int cC, const double fs[5], double dtdx, const GasParams& gp,
+                                 double (*s_r)[NT], int t, int nbefore, int nvalid) {
+    double u[5];
+    load_u(m.unew, m.ncell, cC, u);
+    for (int k = 0; k < nbefore; ++k) {
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] - s_r[k * 5 + iv][t];
+    }
+    own_update(u, fs, dtdx, gp);
+    for (int k = nbefore; k < nvalid; ++k) {
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] - s_r[k * 5 + iv][t];
+    }
+    const size_t ci = (size_t)cC - 1;
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + ci] = u[iv];
+}
+
+// Ordered accumulation of one leaf cell (HydroSolver.cpp:476-547 in
+// processing order).  s_r/s_k hold, for this thread, the six incoming
+// contributions and their keys (negative key: none): sorted by key, those
+// ordered before the cell are subtracted, then the cell's own update and
+// clamps are applied, then the rest.
+template <int NT>
+RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const double fs[5], double dtdx,
+                          const GasParams& gp, double (*s_r)[NT], long long (*s_k)[NT], int t) {
+    const long long KOWN = own_key * 8 + 6;
+    long long ks[6];
+    int ix[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const long long kf = s_k[f][t];
+        ks[f] = kf < 0 ? KMAX : kf;
+        ix[f] = f;
+    }
+    sort6(ks, ix);
     double rs[6][5];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
 #pragma unroll
-        for (int iv = 0; iv < 5; ++iv) {
-            rs[k][iv] = s_r[ix[k] * 5 + iv][t];
-        }
+        for (int iv = 0; iv < 5; ++iv) rs[k][iv] = s_r[ix[k] * 5 + iv][t];
     }
     double u[5];
     load_u(m.unew, m.ncell, cC, u);
@@ -350,35 +419,17 @@ RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const dou
         const bool take = ks[k] < KOWN;
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
-            double nv = u[iv] - rs[k][iv];
+            const double nv = u[iv] - rs[k][iv];
             u[iv] = take ? nv : u[iv];
         }
     }
-    // Own update and clamps, HydroSolver.cpp:489-547.
-#pragma unroll
-    for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] + dtdx * fs[iv];
-    {
-        double d_curr = sclamp(u[0], 1e-10, 1e6);
-        u[0] = d_curr;
-        for (int i = 1; i <= 3; ++i) u[i] = sclamp(u[i], -d_curr * 10.0, d_curr * 10.0);
-        double v2_curr = 0.0;
-        for (int i = 1; i <= 3; ++i) {
-            double v = u[i] / d_curr;
-            v2_curr = v2_curr + v * v;
-        }
-        double ei_curr = u[4] - 0.5 * d_curr * v2_curr;
-        if (ei_curr < d_curr * 1e-10 / gp.gm1) {
-            double e_non = 0;
-            u[4] = d_curr * 1e-10 / gp.gm1 + 0.5 * d_curr * v2_curr + e_non;
-        }
-        u[4] = smin(u[4], d_curr * 1e8);
-    }
+    own_update(u, fs, dtdx, gp);
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         const bool take = ks[k] > KOWN && ks[k] != KMAX;
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
-            double nv = u[iv] - rs[k][iv];
+            const double nv = u[iv] - rs[k][iv];
             u[iv] = take ? nv : u[iv];
         }
     }
@@ -621,8 +672,8 @@ template <int RIEMANN, bool MIXED, bool EXACT>
 __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                                                           double dx, double inv_dx) {
     using TC = typename std::conditional<MIXED, float, double>::type;
+    // Per-thread incoming contributions, stored in processing order.
     __shared__ double s_r[30][O_BLOCK];
-    __shared__ long long s_k[6][O_BLOCK];
     const int t = threadIdx.x;
     const int ic0 = t & 7;
     const int praw = blockIdx.x * O_OCTS + (t >> 3);
@@ -654,6 +705,28 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
     }
     double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
+    // The processing keys of the six incoming contributions follow from the
+    // topology alone, so their order is fixed before any flux is computed:
+    // each contribution is then written straight into its rank's LDS row.
+    int nbefore, nvalid;
+    unsigned rank_pack;
+    {
+        long long keys[6];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const bool par = (ic0 >> d) & 1;
+            const int icS = ic0 ^ (1 << d);
+            const int E = lv.nbo[(size_t)(2 * d + (par ? 1 : 0)) * noct + p];
+            const long long ksib = ((lm >> icS) & 1) ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : KMAX;
+            long long kx = KMAX;
+            if (E >= 0 && ((lv.leaf[E] >> icS) & 1)) kx = ((long long)E * 8 + icS) * 8 + (par ? 2 * d : 2 * d + 1);
+            // The sibling sits on the minus side when par = 1.
+            keys[2 * d] = par ? ksib : kx;
+            keys[2 * d + 1] = par ? kx : ksib;
+        }
+        rank_pack = rank_contributions(keys, (long long)p * 8 + ic0, nbefore, nvalid);
+    }
+
     auto prim = [&](int icell, const TC fallback[5], TC q[5]) {
         if (icell > 0) {
             double u[5], qd[5];
@@ -679,15 +752,18 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
             rc = lv.rec[-1 - E];
             Xcell = rc.x;
         }
-        TC qS[5], qX[5];
-        shfl5(qC, qS, b);
+        // Primitives rotated into the frame of d once; the sibling's come
+        // rotated by shuffle.
+        TC rC[5], rS[5], qX[5], rX[5];
+        rotated(qC, rC, d);
+        shfl5(rC, rS, b);
         prim(Xcell, qC, qX);
-        TC qL1[5], qR1[5];
-        sel5(par, qS, qX, qL1);
-        sel5(par, qX, qS, qR1);
+        rotated(qX, rX, d);
+        TC rL1[5], rR1[5];
+        sel5(par, rS, rX, rL1);
+        sel5(par, rX, rS, rR1);
         TC OCm[5], OCp[5], GCm[5], GCp[5];
-        own_trace(qL1, qC, qR1, d, slope_type, dtc, len, OCm, OCp, gc);
-        ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, len, GCm, GCp, gc);
+        trace_pair(rL1, rC, rR1, slope_type, interpol_type, dtc, len, OCm, OCp, GCm, GCp, gc);
         TC GSm[5], GSp[5];
         shfl5(GCm, GSm, b);
         shfl5(GCp, GSp, b);
@@ -710,17 +786,16 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
         // External face (across the oct boundary).
         double Fext[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         double Fin[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        long long Kin = -1;
         if (E >= 0) {
             if ((lv.leaf[E] >> icS) & 1) {
-                TC qX2[5];
+                TC qX2[5], rX2[5];
                 prim(cell_id(m, lv.grid[E], ic0), qX, qX2);
+                rotated(qX2, rX2, d);
                 TC a5[5], c5[5];
-                sel5(par, qC, qX2, a5);
-                sel5(par, qX2, qC, c5);
+                sel5(par, rC, rX2, a5);
+                sel5(par, rX2, rC, c5);
                 TC GXm[5], GXp[5], OXm[5], OXp[5];
-                ghost_trace(a5, qX, c5, d, interpol_type, dtc, len, GXm, GXp, gc);
-                own_trace(a5, qX, c5, d, slope_type, dtc, len, OXm, OXp, gc);
+                trace_pair(a5, rX, c5, slope_type, interpol_type, dtc, len, OXm, OXp, GXm, GXp, gc);
                 TC ql[5], qr[5];
                 sel5(par, OCm, GXm, ql);
                 sel5(par, GXp, OCp, qr);
@@ -730,7 +805,6 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
                 sel5(par, OXp, GCp, qr);
                 solve_face<RIEMANN, MIXED>(ql, qr, Fin, gc, gd);
                 rotate(Fin, d);
-                Kin = ((long long)E * 8 + icS) * 8 + (par ? 2 * d : 2 * d + 1);
             }
         } else if (rc.w & REC_REFINED_DEV) {
         } else if (rc.w & REC_BOUNDARY_DEV) {
@@ -784,16 +858,16 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
         const int slotX = 2 * d + (par ? 1 : 0);
         const double sgS = par ? -1.0 : 1.0;
         const double sgX = par ? 1.0 : -1.0;
+        const int rowS = 5 * ((rank_pack >> (3 * slotS)) & 7);
+        const int rowX = 5 * ((rank_pack >> (3 * slotX)) & 7);
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
-            s_r[slotS * 5 + iv][t] = dtdx * sgS * Fsib[iv] * 0.125;
-            s_r[slotX * 5 + iv][t] = dtdx * sgX * Fin[iv] * 0.125;
+            s_r[rowS + iv][t] = dtdx * sgS * Fsib[iv] * 0.125;
+            s_r[rowX + iv][t] = dtdx * sgX * Fin[iv] * 0.125;
         }
-        s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
-        s_k[slotX][t] = Kin;
     }
     if (!leaf) return;
-    ordered_apply<O_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t);
+    ordered_apply_ranked<O_BLOCK>(m, cC, fs, dtdx, gp, s_r, t, nbefore, nvalid);
 }
 
 // Reflux into coarser leaf cells, one thread per target cell, contributions
@@ -814,33 +888,30 @@ __global__ __launch_bounds__(256) void k_coarse_reflux(MeshDev m, LevelDev lv) {
     for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
 }
 
-// HydroSolver.cpp:734-876 for nvar = 5, one thread per new oct.
-__global__ __launch_bounds__(64) void k_interpol(int n_octs, const double* __restrict__ u1,
-                                                 double* __restrict__ u2, GasParams gp, int interpol_var) {
-    const int o = blockIdx.x * 64 + threadIdx.x;
-    if (o >= n_octs) return;
-    const double* U1 = u1 + (size_t)o * 7 * 64;
-    double* U2 = u2 + (size_t)o * 8 * 64;
+// HydroSolver.cpp:734-876 for nvar = 5: prolongation of one coarse cell and
+// its six neighbours (U1(i, iv), i = 0..6) into the eight children
+// (U2(i, iv, value), i = 0..7).
+template <class In, class Out>
+RH_DEV void interpol_oct(In U1, Out U2, const GasParams& gp, int interpol_var) {
     const Gas<double> g{gp.gamma, gp.gm1, gp.smalle, gp.smallp, 1.0 / gp.gm1};
     const int it = gp.interpol_type;
     double q1[7][5];
     for (int i = 0; i < 7; ++i) {
-        const double* ui = U1 + i * 64;
         if (interpol_var == 0) {
-            for (int iv = 0; iv < 5; ++iv) q1[i][iv] = ui[iv];
+            for (int iv = 0; iv < 5; ++iv) q1[i][iv] = U1(i, iv);
         } else if (interpol_var == 1) {
-            double d = smax(ui[0], 1e-10);
-            q1[i][0] = ui[0];
+            double d = smax(U1(i, 0), 1e-10);
+            q1[i][0] = U1(i, 0);
             double e_kin = 0.0;
             for (int k = 1; k <= 3; ++k) {
-                q1[i][k] = ui[k];
-                e_kin = e_kin + 0.5 * ui[k] * ui[k] / d;
+                q1[i][k] = U1(i, k);
+                e_kin = e_kin + 0.5 * U1(i, k) * U1(i, k) / d;
             }
             double e_nonthermal = 0.0;
-            q1[i][4] = ui[4] - e_kin - e_nonthermal;
+            q1[i][4] = U1(i, 4) - e_kin - e_nonthermal;
         } else {
             double uu[5];
-            for (int iv = 0; iv < 5; ++iv) uu[iv] = ui[iv];
+            for (int iv = 0; iv < 5; ++iv) uu[iv] = U1(i, iv);
             ctoprim(uu, q1[i], g);
         }
     }
@@ -867,25 +938,25 @@ __global__ __launch_bounds__(64) void k_interpol(int n_octs, const double* __res
     }
     if (interpol_var == 0) {
         for (int i = 0; i < 8; ++i)
-            for (int iv = 0; iv < 5; ++iv) U2[i * 64 + iv] = q2[i][iv];
+            for (int iv = 0; iv < 5; ++iv) U2(i, iv, q2[i][iv]);
     } else if (interpol_var == 1) {
         for (int i = 0; i < 8; ++i) {
             double d = smax(q2[i][0], 1e-10);
-            U2[i * 64 + 0] = d;
+            U2(i, 0, d);
             double e_kin = 0.0;
             for (int k = 1; k <= 3; ++k) {
-                U2[i * 64 + k] = q2[i][k];
+                U2(i, k, q2[i][k]);
                 e_kin = e_kin + 0.5 * q2[i][k] * q2[i][k] / d;
             }
             double e_nonthermal = 0.0;
-            U2[i * 64 + 4] = q2[i][4] + e_kin + e_nonthermal;
+            U2(i, 4, q2[i][4] + e_kin + e_nonthermal);
         }
     } else {
         const double oneover_n2d = 1.0 / 8;
         for (int k = 1; k <= 3; ++k) {
             double mom_sum = 0.0;
             for (int i = 0; i < 8; ++i) mom_sum = mom_sum + q2[i][0] * q2[i][k] * oneover_n2d;
-            double mom_err = mom_sum - U1[k];
+            double mom_err = mom_sum - U1(0, k);
             for (int i = 0; i < 8; ++i) {
                 double d = smax(q2[i][0], 1e-10);
                 q2[i][k] = (d * q2[i][k] - mom_err) / d;
@@ -894,16 +965,142 @@ __global__ __launch_bounds__(64) void k_interpol(int n_octs, const double* __res
         for (int i = 0; i < 8; ++i) {
             double d = smax(q2[i][0], 1e-10);
             double pp = smax(q2[i][4], 1e-20);
-            U2[i * 64 + 0] = d;
+            U2(i, 0, d);
             double v2 = 0;
             for (int k = 1; k <= 3; ++k) {
-                U2[i * 64 + k] = d * q2[i][k];
+                U2(i, k, d * q2[i][k]);
                 v2 = v2 + q2[i][k] * q2[i][k];
             }
             double e_thermal = pp / g.gm1, e_kinetic = 0.5 * d * v2, e_nonthermal = 0.0;
-            U2[i * 64 + 4] = e_thermal + e_kinetic + e_nonthermal;
+            U2(i, 4, e_thermal + e_kinetic + e_nonthermal);
         }
     }
+}
+
+// interpol_hydro over independent [7][64] -> [8][64] blocks, one thread each.
+__global__ __launch_bounds__(64) void k_interpol(int n_octs, const double* __restrict__ u1,
+                                                 double* __restrict__ u2, GasParams gp, int interpol_var) {
+    const int o = blockIdx.x * 64 + threadIdx.x;
+    if (o >= n_octs) return;
+    const double* U1 = u1 + (size_t)o * 7 * 64;
+    double* U2 = u2 + (size_t)o * 8 * 64;
+    interpol_oct([&](int i, int iv) { return U1[i * 64 + iv]; },
+                 [&](int i, int iv, double v) { U2[i * 64 + iv] = v; }, gp, interpol_var);
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive-run kernels (the tree itself is edited on the host).
+
+// New octs of make_grid_fine (TreeUpdater.cpp:115-147): for oct o, cells[8o]
+// is the refined cell, cells[8o+1..8o+6] its six neighbours (<= 0: use the
+// cell itself; no reflective boundaries exist), cells[8o+7] the new grid.
+// The eight children get the prolongated state in uold and unew.
+__global__ __launch_bounds__(64) void k_new_octs(MeshDev m, int n, const int* __restrict__ cells, GasParams gp,
+                                                 int interpol_var) {
+    const int o = blockIdx.x * 64 + threadIdx.x;
+    if (o >= n) return;
+    const int* c = cells + (size_t)o * 8;
+    const int grid = c[7];
+    interpol_oct(
+        [&](int i, int iv) {
+            const int id = c[i] > 0 ? c[i] : c[0];
+            return m.uold[(size_t)iv * m.ncell + (size_t)id - 1];
+        },
+        [&](int i, int iv, double v) {
+            const size_t id = (size_t)cell_id(m, grid, i) - 1;
+            m.uold[(size_t)iv * m.ncell + id] = v;
+            m.unew[(size_t)iv * m.ncell + id] = v;
+        },
+        gp, interpol_var);
+}
+
+// restrict_fine (TreeUpdater.cpp:195-215): the father cell of every oct of
+// the finer list gets the mean of its eight children's uold.
+__global__ __launch_bounds__(256) void k_restrict(MeshDev m, LevelDev fine, const int* __restrict__ father) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= fine.noct) return;
+    const int g = fine.grid[p];
+    const int fc = father[g - 1];
+    if (fc <= 0) return;
+    for (int iv = 0; iv < 5; ++iv) {
+        double sum = 0;
+        for (int ic = 0; ic < 8; ++ic) sum += m.uold[(size_t)iv * m.ncell + (size_t)cell_id(m, g, ic) - 1];
+        m.uold[(size_t)iv * m.ncell + (size_t)fc - 1] = sum / 8;
+    }
+}
+
+struct FlagState {
+    double d, v[3], p;
+};
+
+// TreeUpdater.cpp:342-384 (hydro, nener = 0); zeros outside 1..ncell.
+RH_DEV FlagState flag_state(const MeshDev& m, int id, double gamma) {
+    FlagState s{0.0, {0.0, 0.0, 0.0}, 0.0};
+    if (id <= 0 || id > m.ncell) return s;
+    double u[5];
+    load_u(m.uold, m.ncell, id, u);
+    s.d = smax(u[0], 1e-10);
+    double v2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        s.v[k] = u[1 + k] / s.d;
+        v2 += s.v[k] * s.v[k];
+    }
+    double ek = 0.5 * s.d * v2;
+    double em = 0.0, e_nonthermal = 0.0;
+    s.p = smax((u[4] - ek - em - e_nonthermal) * (gamma - 1.0), s.d * 1e-10);
+    return s;
+}
+
+RH_DEV double err_grad(double l, double c, double r, double floor) {
+    return 2.0 * smax(sabs(r - c) / (r + c + floor), sabs(c - l) / (c + l + floor));
+}
+
+RH_DEV double err_grad_u(double vl, double vc, double vr, double pl, double pc, double pr, double dl, double dc,
+                         double dr, double gamma) {
+    const double floor = 1e-10;
+    const double cg = ssqrt(smax(gamma * pl / dl, floor * floor));
+    const double cm = ssqrt(smax(gamma * pc / dc, floor * floor));
+    const double cd = ssqrt(smax(gamma * pr / dr, floor * floor));
+    return 2.0 * smax(sabs(vr - vc) / (cd + cm + sabs(vr) + sabs(vc) + floor),
+                      sabs(vc - vl) / (cm + cg + sabs(vc) + sabs(vl) + floor));
+}
+
+// Gradient criteria of flag_fine (TreeUpdater.cpp:482-526) for every cell of
+// a level list: out[8*pos + ic-1] = 1 when a density, pressure or velocity
+// error exceeds its threshold along some direction.  The neighbour along a
+// face is the same-level cell when its oct exists (get_nbor_cells_exact),
+// else the grid's nbor entry; a neighbour outside 1..ncell is replaced by
+// the cell itself.
+__global__ __launch_bounds__(256) void k_grad_flags(MeshDev m, LevelDev lv, const int* __restrict__ nbor,
+                                                    double ed, double ep, double ev, double gamma,
+                                                    unsigned char* __restrict__ out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= lv.noct * 8) return;
+    const int p = k >> 3, ic0 = k & 7;
+    const int g = lv.grid[p];
+    const int c = cell_id(m, g, ic0);
+    const FlagState sc = flag_state(m, c, gamma);
+    bool ok = false;
+    for (int d = 0; d < 3 && !ok; ++d) {
+        int idn[2];
+        for (int side = 0; side < 2; ++side) {
+            const int par = (ic0 >> d) & 1;
+            if (par != side) {
+                idn[side] = cell_id(m, g, ic0 ^ (1 << d));  // sibling inside the oct
+            } else {
+                const int E = lv.nbo[(size_t)(2 * d + side) * lv.noct + p];
+                idn[side] = E >= 0 ? cell_id(m, lv.grid[E], ic0 ^ (1 << d))
+                                   : nbor[(size_t)(2 * d + side) * m.ngridmax + g - 1];
+            }
+        }
+        const FlagState sl = (idn[0] > 0 && idn[0] <= m.ncell) ? flag_state(m, idn[0], gamma) : sc;
+        const FlagState sr = (idn[1] > 0 && idn[1] <= m.ncell) ? flag_state(m, idn[1], gamma) : sc;
+        if (ed > 0.0 && err_grad(sl.d, sc.d, sr.d, 1e-10) > ed) ok = true;
+        else if (ep > 0.0 && err_grad(sl.p, sc.p, sr.p, 1e-10) > ep) ok = true;
+        else if (ev > 0.0 && err_grad_u(sl.v[d], sc.v[d], sr.v[d], sl.p, sc.p, sr.p, sl.d, sc.d, sr.d, gamma) > ev)
+            ok = true;
+    }
+    out[k] = ok ? 1 : 0;
 }
 
 inline int blocks_for(int noct) { return (noct + OCTS_PER_BLOCK - 1) / OCTS_PER_BLOCK; }
@@ -1010,6 +1207,28 @@ hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams&
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || lv.ntgt == 0) return e;
     hipLaunchKernelGGL(k_coarse_reflux, dim3((lv.ntgt + 255) / 256), dim3(256), 0, stream, m, lv);
+    return hipGetLastError();
+}
+
+hipError_t launch_new_octs(const MeshDev& m, int n, const int* d_cells, const GasParams& gp, int interpol_var,
+                           hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_new_octs, dim3((n + 63) / 64), dim3(64), 0, stream, m, n, d_cells, gp, interpol_var);
+    return hipGetLastError();
+}
+
+hipError_t launch_restrict(const MeshDev& m, const LevelDev& fine, const int* d_father, hipStream_t stream) {
+    if (fine.noct == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_restrict, dim3((fine.noct + 255) / 256), dim3(256), 0, stream, m, fine, d_father);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_flags(const MeshDev& m, const LevelDev& lv, const int* d_nbor, double ed, double ep,
+                             double ev, double gamma, unsigned char* d_out, hipStream_t stream) {
+    if (lv.noct == 0) return hipSuccess;
+    const int n = lv.noct * 8;
+    hipLaunchKernelGGL(k_grad_flags, dim3((n + 255) / 256), dim3(256), 0, stream, m, lv, d_nbor, ed, ep, ev, gamma,
+                       d_out);
     return hipGetLastError();
 }
 

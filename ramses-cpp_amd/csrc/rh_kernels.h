@@ -78,6 +78,15 @@ hipError_t launch_courant_boxes(const MeshDev& m, const LevelDev& lv, double gam
 hipError_t launch_fill_keys(unsigned long long* keys, int n, double value, hipStream_t stream);
 // dt[i] <- the double encoded by keys[i], i < n.
 hipError_t launch_keys_to_dt(const unsigned long long* keys, int n, double* dt, hipStream_t stream);
+// Adaptive run: prolongation into new octs (d_cells: 8 ints per new oct,
+// the refined cell, its six neighbours, the new grid), restriction of a list
+// into its fathers, and the gradient refinement criteria of one list
+// (d_out[8*pos + ic-1]).
+hipError_t launch_new_octs(const MeshDev& m, int n, const int* d_cells, const GasParams& gp, int interpol_var,
+                           hipStream_t stream);
+hipError_t launch_restrict(const MeshDev& m, const LevelDev& fine, const int* d_father, hipStream_t stream);
+hipError_t launch_grad_flags(const MeshDev& m, const LevelDev& lv, const int* d_nbor, double ed, double ep,
+                             double ev, double gamma, unsigned char* d_out, hipStream_t stream);
 hipError_t launch_interpol(int n_octs, const double* d_u1, double* d_u2, const GasParams& gp, int interpol_var,
                            hipStream_t stream);
 

@@ -156,18 +156,14 @@ template <typename T> RH_DEV T ghost_limiter(T dlft, T drgt, int interpol_type) 
     return s;
 }
 
-// MUSCL-Hancock predictor in a rotated frame (HydroSolver.cpp:680-732).
+// MUSCL-Hancock predictor in a rotated frame (HydroSolver.cpp:680-732), with
+// the cell's density floor r, normal velocity u and sound speed squared c2
+// given (shared by every trace of the same cell).
 template <typename T>
-RH_DEV void trace(const T q[5], const T dq[5], T dt, T dx, T qm[5], T qp[5], const Gas<T>& g) {
-    T r = smax(q[0], T(1e-10));
-    T u = q[1];
-    T p = smax(q[4], r * T(1e-20));
+RH_DEV void trace_core(const T q[5], const T dq[5], T r, T u, T c2, T hdx, T hdt, T qm[5], T qp[5]) {
     T sr0 = -u * dq[0] - dq[1] * r;
     T su0 = -u * dq[1] - dv(dq[4], r);
-    T c2 = cs2(r, p, g);
     T sp0 = -u * dq[4] - dq[1] * r * c2;
-    T hdx = T(0.5) * dx;
-    T hdt = T(0.5) * dt;
     T src[5] = {sr0, su0, T(0), T(0), sp0};
 #pragma unroll
     for (int iv = 0; iv < 5; ++iv) {
@@ -178,23 +174,19 @@ RH_DEV void trace(const T q[5], const T dq[5], T dt, T dx, T qm[5], T qp[5], con
     if (qm[0] < T(1e-10)) qm[0] = r;
 }
 
-// Own face states of a cell along dimension d (HydroSolver.cpp:93-115 with
-// compute_slopes :653-678).  ql/qc/qr are the unrotated primitives of the
-// left neighbour, the cell and the right neighbour; outputs are rotated.
+template <typename T>
+RH_DEV void trace(const T q[5], const T dq[5], T dt, T dx, T qm[5], T qp[5], const Gas<T>& g) {
+    T r = smax(q[0], T(1e-10));
+    T p = smax(q[4], r * T(1e-20));
+    trace_core(q, dq, r, q[1], cs2(r, p, g), T(0.5) * dx, T(0.5) * dt, qm, qp);
+}
+
+// Slopes of the trace pass (compute_slopes, HydroSolver.cpp:653-678) from the
+// rotated primitives of the left neighbour a, the cell b, the right
+// neighbour c.
 template <typename T, bool E>
-RH_DEV void own_trace(const T ql[5], const T qc[5], const T qr[5], int d, int slope_type, T dt, const Len<T, E>& len,
-                      T qm[5], T qp[5], const Gas<T>& g) {
-    T a[5], b[5], c[5];
-#pragma unroll
-    for (int iv = 0; iv < 5; ++iv) {
-        a[iv] = ql[iv];
-        b[iv] = qc[iv];
-        c[iv] = qr[iv];
-    }
-    rotate(a, d);
-    rotate(b, d);
-    rotate(c, d);
-    T dq[5];
+RH_DEV void own_slopes(const T a[5], const T b[5], const T c[5], int slope_type, T dt, const Len<T, E>& len,
+                       T dq[5]) {
     if (slope_type == 0) {
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) dq[iv] = T(0.0);
@@ -206,6 +198,64 @@ RH_DEV void own_trace(const T ql[5], const T qc[5], const T qr[5], int d, int sl
             else dq[iv] = over(limiter(a[iv], b[iv], c[iv], slope_type, nu), len);
         }
     }
+}
+
+// Interpolation slopes of a coarse-ghost state (HydroSolver.cpp:391-404) over
+// the coarse width in lc, from rotated primitives.
+template <typename T, bool E>
+RH_DEV void ghost_slopes(const T a[5], const T b[5], const T c[5], int interpol_type, const Len<T, E>& lc, T dq[5]) {
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) dq[iv] = over(ghost_limiter(b[iv] - a[iv], c[iv] - b[iv], interpol_type), lc);
+}
+
+// Both face-state pairs of one cell from rotated primitives: its own states
+// (trace pass, HydroSolver.cpp:93-115: slope_type slopes over dx) and its
+// ghost states as a coarser neighbour's (HydroSolver.cpp:357-440:
+// interpol_type slopes over 2*dx).  The two traces share the cell's floors
+// and sound speed.
+template <typename T, bool E>
+RH_DEV void trace_pair(const T a[5], const T b[5], const T c[5], int slope_type, int interpol_type, T dt,
+                       const Len<T, E>& len, T om[5], T op[5], T gm[5], T gp[5], const Gas<T>& g) {
+    T dqo[5], dqg[5];
+    own_slopes(a, b, c, slope_type, dt, len, dqo);
+    const Len<T, E> lc = coarse(len);
+    ghost_slopes(a, b, c, interpol_type, lc, dqg);
+    const T r = smax(b[0], T(1e-10));
+    const T p = smax(b[4], r * T(1e-20));
+    const T c2 = cs2(r, p, g);
+    const T hdt = T(0.5) * dt;
+    trace_core(b, dqo, r, b[1], c2, T(0.5) * len.dx, hdt, om, op);
+    trace_core(b, dqg, r, b[1], c2, T(0.5) * lc.dx, hdt, gm, gp);
+}
+
+// Ghost face states only (a coarser neighbour's), rotated primitives.
+template <typename T, bool E>
+RH_DEV void ghost_trace_rot(const T a[5], const T b[5], const T c[5], int interpol_type, T dt, const Len<T, E>& len,
+                            T qm[5], T qp[5], const Gas<T>& g) {
+    const Len<T, E> lc = coarse(len);
+    T dq[5];
+    ghost_slopes(a, b, c, interpol_type, lc, dq);
+    trace(b, dq, dt, lc.dx, qm, qp, g);
+}
+
+template <typename T> RH_DEV void rotated(const T q[5], T out[5], int d) {
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) out[iv] = q[iv];
+    rotate(out, d);
+}
+
+// Own face states of a cell along dimension d (HydroSolver.cpp:93-115 with
+// compute_slopes :653-678).  ql/qc/qr are the unrotated primitives of the
+// left neighbour, the cell and the right neighbour; outputs are rotated.
+template <typename T, bool E>
+RH_DEV void own_trace(const T ql[5], const T qc[5], const T qr[5], int d, int slope_type, T dt, const Len<T, E>& len,
+                      T qm[5], T qp[5], const Gas<T>& g) {
+    T a[5], b[5], c[5];
+    rotated(ql, a, d);
+    rotated(qc, b, d);
+    rotated(qr, c, d);
+    T dq[5];
+    own_slopes(a, b, c, slope_type, dt, len, dq);
     trace(b, dq, dt, len.dx, qm, qp, g);
 }
 
@@ -216,20 +266,10 @@ template <typename T, bool E>
 RH_DEV void ghost_trace(const T ql[5], const T qc[5], const T qr[5], int d, int interpol_type, T dt,
                         const Len<T, E>& len, T qm[5], T qp[5], const Gas<T>& g) {
     T a[5], b[5], c[5];
-#pragma unroll
-    for (int iv = 0; iv < 5; ++iv) {
-        a[iv] = ql[iv];
-        b[iv] = qc[iv];
-        c[iv] = qr[iv];
-    }
-    rotate(a, d);
-    rotate(b, d);
-    rotate(c, d);
-    const Len<T, E> lc = coarse(len);
-    T dq[5];
-#pragma unroll
-    for (int iv = 0; iv < 5; ++iv) dq[iv] = over(ghost_limiter(b[iv] - a[iv], c[iv] - b[iv], interpol_type), lc);
-    trace(b, dq, dt, lc.dx, qm, qp, g);
+    rotated(ql, a, d);
+    rotated(qc, b, d);
+    rotated(qr, c, d);
+    ghost_trace_rot(a, b, c, interpol_type, dt, len, qm, qp, g);
 }
 
 // Ghost of a boundary face: the cell's own state, normal momentum flipped for

@@ -60,6 +60,21 @@ def resolve_riemann(name: str | None) -> int:
     return RIEMANN.get(name, 0) if name in ("hllc", "hll", "exact", "acoustic") else 0
 
 
+class RhAmrParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ngridmax", C.c_int),
+        ("levelmin", C.c_int), ("levelmax", C.c_int), ("courant_factor", C.c_double),
+        ("err_grad_d", C.c_double), ("err_grad_p", C.c_double), ("err_grad_v", C.c_double),
+        ("nexpand", C.c_int), ("ic_kind", C.c_int), ("nreg", C.c_int), ("reg_type", C.POINTER(C.c_int)),
+        ("x_center", C.POINTER(C.c_double)), ("y_center", C.POINTER(C.c_double)),
+        ("z_center", C.POINTER(C.c_double)), ("length_x", C.POINTER(C.c_double)),
+        ("length_y", C.POINTER(C.c_double)), ("length_z", C.POINTER(C.c_double)),
+        ("d_region", C.POINTER(C.c_double)), ("u_region", C.POINTER(C.c_double)),
+        ("v_region", C.POINTER(C.c_double)), ("w_region", C.POINTER(C.c_double)),
+        ("p_region", C.POINTER(C.c_double)), ("kh_amp", C.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -97,6 +112,13 @@ def lib() -> C.CDLL:
         L.rh_autotune.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int), _DP]
         L.rh_add_gravity_source_terms.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.rh_synchro_hydro_fine.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.rh_amr_create.argtypes = [C.POINTER(RhParams), C.POINTER(RhAmrParams), C.POINTER(C.c_void_p)]
+        L.rh_amr_destroy.argtypes = [C.c_void_p]
+        L.rh_amr_step.argtypes = [C.c_void_p, C.c_int]
+        L.rh_amr_mesh.argtypes = [C.c_void_p, C.POINTER(RhMesh), C.POINTER(_IP), C.POINTER(_DP)]
+        L.rh_amr_stats.argtypes = [C.c_void_p, _DP, _IP, C.POINTER(C.c_longlong)]
+        L.rh_amr_context.argtypes = [C.c_void_p]
+        L.rh_amr_context.restype = C.c_void_p
         L.rh_init_shock_tube.argtypes = [C.POINTER(RhMesh), _DP, C.c_double, C.c_double, _DP, _DP, _DP, _DP]
         L.rh_bind_ensemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhMesh), C.c_int]
         L.rh_run_ensemble.argtypes = [C.c_void_p, _IP, _DP, C.c_double, C.c_int, _DP]
@@ -111,7 +133,8 @@ EXPORTED = [
     "rh_init_regions", "rh_init_lognormal", "rh_timers_enable", "rh_timers_read", "rh_timers_reset",
     "rh_level_leaf_cells", "rh_set_kernel_variant", "rh_get_kernel_variant", "rh_autotune", "rh_init_shock_tube",
     "rh_bind_ensemble", "rh_run_ensemble", "rh_ensemble_leaf_cells", "rh_add_gravity_source_terms",
-    "rh_synchro_hydro_fine",
+    "rh_synchro_hydro_fine", "rh_amr_create", "rh_amr_destroy", "rh_amr_step", "rh_amr_mesh", "rh_amr_stats",
+    "rh_amr_context",
 ]
 
 
@@ -432,6 +455,91 @@ class Solver:
         if self._ctx:
             lib().rh_destroy(self._ctx)
             self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class AmrRun:
+    """An adaptive run with the state on the GPU (rhydro_amr.h): built as
+    Simulation::initialize builds it, then advanced by coarse steps."""
+
+    def __init__(self, nx=1, levelmin=4, levelmax=6, ngridmax=20000, gamma=1.4, courant_factor=0.8,
+                 err_grad_d=-1.0, err_grad_p=-1.0, err_grad_v=-1.0, nexpand=1, slope_type=1, riemann="hllc",
+                 interpol_type=1, interpol_var=0, boxlen=1.0, regions=None, ic="regions", kh_amp=0.01,
+                 precision=RH_PREC_FP64, device=0):
+        regions = regions or []
+        self._keep = []
+
+        def col(key, default):
+            a = np.array([float(r.get(key, default)) for r in regions] or [0.0], np.float64)
+            self._keep.append(a)
+            return _dp(a)
+
+        types = np.array([1 if r.get("type", "cube") == "sphere" else 0 for r in regions] or [0], np.int32)
+        self._keep.append(types)
+        rv = riemann if isinstance(riemann, int) else resolve_riemann(riemann)
+        self.params = RhParams(gamma, slope_type, rv, interpol_type, interpol_var, 5, 0, 0, boxlen, precision,
+                               device)
+        ap = RhAmrParams()
+        ap.nx = ap.ny = ap.nz = nx
+        ap.ngridmax, ap.levelmin, ap.levelmax = ngridmax, levelmin, levelmax
+        ap.courant_factor = courant_factor
+        ap.err_grad_d, ap.err_grad_p, ap.err_grad_v = err_grad_d, err_grad_p, err_grad_v
+        ap.nexpand = nexpand
+        ap.ic_kind = 1 if ic == "kh" else 0
+        ap.nreg = len(regions)
+        ap.reg_type = _ip(types)
+        (ap.x_center, ap.y_center, ap.z_center, ap.length_x, ap.length_y, ap.length_z, ap.d_region, ap.u_region,
+         ap.v_region, ap.w_region, ap.p_region) = (
+            col("x_center", 0.5), col("y_center", 0.5), col("z_center", 0.5), col("length_x", 1.0),
+            col("length_y", 1.0), col("length_z", 1.0), col("d", 1.0), col("u", 0.0), col("v", 0.0),
+            col("w", 0.0), col("p", 1.0))
+        ap.kh_amp = kh_amp
+        self._h = C.c_void_p()
+        st = lib().rh_amr_create(C.byref(self.params), C.byref(ap), C.byref(self._h))
+        if st != RH_OK:
+            raise RhError(st, lib().rh_last_error(None).decode())
+        self.nlevelmax = levelmax
+
+    def _chk(self, st):
+        if st != RH_OK:
+            raise RhError(st, lib().rh_last_error(lib().rh_amr_context(self._h)).decode())
+
+    def step(self, n=1):
+        self._chk(lib().rh_amr_step(self._h, n))
+
+    def stats(self):
+        t, n, cu = C.c_double(), C.c_int(), C.c_longlong()
+        self._chk(lib().rh_amr_stats(self._h, C.byref(t), C.byref(n), C.byref(cu)))
+        return t.value, n.value, cu.value
+
+    def state(self) -> dict:
+        """Copies of the run's arrays (state copied back from the GPU)."""
+        m = RhMesh()
+        nb, xg = _IP(), _DP()
+        self._chk(lib().rh_amr_mesh(self._h, C.byref(m), C.byref(nb), C.byref(xg)))
+        ng, nc, nl = m.ngridmax, m.ncell, m.nlevelmax
+
+        def arr(p, n):
+            return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+        t, nstep, cu = self.stats()
+        return dict(ncoarse=m.ncoarse, ngridmax=ng, ncell=nc, nlevelmax=nl, father=arr(m.father, ng),
+                    son=arr(m.son, nc), nbor=arr(m.nbor, 6 * ng), next=arr(m.next, ng), headl=arr(m.headl, nl + 1),
+                    numbl=arr(nb, nl + 1), xg=arr(xg, 3 * ng), uold=arr(m.uold, 5 * nc), unew=arr(m.unew, 5 * nc),
+                    t=t, nstep=nstep, cell_updates=cu)
+
+    def solver_context(self):
+        return lib().rh_amr_context(self._h)
+
+    def close(self):
+        if self._h:
+            lib().rh_amr_destroy(self._h)
+            self._h = C.c_void_p()
 
     def __del__(self):
         try:
