@@ -668,12 +668,27 @@ template <typename T> RH_DEV void sel5(bool c, const T a[5], const T b[5], T out
 constexpr int O_OCTS = 16;
 constexpr int O_BLOCK = O_OCTS * 8;
 
+// Occupancy: the mixed form with HLLC or the acoustic solver fits three
+// waves per SIMD in 168 VGPRs (a few spilled in cold paths) once its LDS
+// payload drops the keys: contributions are written in processing order
+// (RANKED) and 30 KB of LDS per workgroup lets five workgroups share a CU
+// (measured 3.5% faster at 256^3 for HLLC).  With LLF or HLL the same cap
+// spills in the hot path and runs slower, and the FP64 form is register-bound
+// at two waves either way; those keep the keys beside the contributions and
+// sort them at the end.
+template <int R, bool M>
+constexpr bool oct_ranked() {
+    return M && (R == 2 || R == 4);
+}
+
 template <int RIEMANN, bool MIXED, bool EXACT>
-__global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
+__global__ __launch_bounds__(O_BLOCK) __attribute__((amdgpu_waves_per_eu(oct_ranked<RIEMANN, MIXED>() ? 3 : 1, 8)))
+void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                                                           double dx, double inv_dx) {
     using TC = typename std::conditional<MIXED, float, double>::type;
-    // Per-thread incoming contributions, stored in processing order.
+    constexpr bool RANKED = oct_ranked<RIEMANN, MIXED>();
     __shared__ double s_r[30][O_BLOCK];
+    __shared__ long long s_k[RANKED ? 1 : 6][O_BLOCK];
     const int t = threadIdx.x;
     const int ic0 = t & 7;
     const int praw = blockIdx.x * O_OCTS + (t >> 3);
@@ -705,12 +720,12 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
     }
     double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
-    // The processing keys of the six incoming contributions follow from the
-    // topology alone, so their order is fixed before any flux is computed:
-    // each contribution is then written straight into its rank's LDS row.
-    int nbefore, nvalid;
-    unsigned rank_pack;
-    {
+    // Ranked form: the processing keys of the six incoming contributions
+    // follow from the topology alone, so their order is fixed before any flux
+    // is computed and each contribution goes straight into its rank's row.
+    int nbefore = 0, nvalid = 0;
+    unsigned rank_pack = 0;
+    if constexpr (RANKED) {
         long long keys[6];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -752,18 +767,15 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
             rc = lv.rec[-1 - E];
             Xcell = rc.x;
         }
-        // Primitives rotated into the frame of d once; the sibling's come
-        // rotated by shuffle.
-        TC rC[5], rS[5], qX[5], rX[5];
-        rotated(qC, rC, d);
-        shfl5(rC, rS, b);
+        TC qS[5], qX[5];
+        shfl5(qC, qS, b);
         prim(Xcell, qC, qX);
-        rotated(qX, rX, d);
-        TC rL1[5], rR1[5];
-        sel5(par, rS, rX, rL1);
-        sel5(par, rX, rS, rR1);
+        TC qL1[5], qR1[5];
+        sel5(par, qS, qX, qL1);
+        sel5(par, qX, qS, qR1);
         TC OCm[5], OCp[5], GCm[5], GCp[5];
-        trace_pair(rL1, rC, rR1, slope_type, interpol_type, dtc, len, OCm, OCp, GCm, GCp, gc);
+        own_trace(qL1, qC, qR1, d, slope_type, dtc, len, OCm, OCp, gc);
+        ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, len, GCm, GCp, gc);
         TC GSm[5], GSp[5];
         shfl5(GCm, GSm, b);
         shfl5(GCp, GSp, b);
@@ -786,16 +798,17 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
         // External face (across the oct boundary).
         double Fext[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         double Fin[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        long long Kin = -1;
         if (E >= 0) {
             if ((lv.leaf[E] >> icS) & 1) {
-                TC qX2[5], rX2[5];
+                TC qX2[5];
                 prim(cell_id(m, lv.grid[E], ic0), qX, qX2);
-                rotated(qX2, rX2, d);
                 TC a5[5], c5[5];
-                sel5(par, rC, rX2, a5);
-                sel5(par, rX2, rC, c5);
+                sel5(par, qC, qX2, a5);
+                sel5(par, qX2, qC, c5);
                 TC GXm[5], GXp[5], OXm[5], OXp[5];
-                trace_pair(a5, rX, c5, slope_type, interpol_type, dtc, len, OXm, OXp, GXm, GXp, gc);
+                ghost_trace(a5, qX, c5, d, interpol_type, dtc, len, GXm, GXp, gc);
+                own_trace(a5, qX, c5, d, slope_type, dtc, len, OXm, OXp, gc);
                 TC ql[5], qr[5];
                 sel5(par, OCm, GXm, ql);
                 sel5(par, GXp, OCp, qr);
@@ -805,6 +818,7 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
                 sel5(par, OXp, GCp, qr);
                 solve_face<RIEMANN, MIXED>(ql, qr, Fin, gc, gd);
                 rotate(Fin, d);
+                Kin = ((long long)E * 8 + icS) * 8 + (par ? 2 * d : 2 * d + 1);
             }
         } else if (rc.w & REC_REFINED_DEV) {
         } else if (rc.w & REC_BOUNDARY_DEV) {
@@ -858,16 +872,21 @@ __global__ __launch_bounds__(O_BLOCK) void k_godunov_oct(MeshDev m, LevelDev lv,
         const int slotX = 2 * d + (par ? 1 : 0);
         const double sgS = par ? -1.0 : 1.0;
         const double sgX = par ? 1.0 : -1.0;
-        const int rowS = 5 * ((rank_pack >> (3 * slotS)) & 7);
-        const int rowX = 5 * ((rank_pack >> (3 * slotX)) & 7);
+        const int rowS = RANKED ? 5 * ((rank_pack >> (3 * slotS)) & 7) : 5 * slotS;
+        const int rowX = RANKED ? 5 * ((rank_pack >> (3 * slotX)) & 7) : 5 * slotX;
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
             s_r[rowS + iv][t] = dtdx * sgS * Fsib[iv] * 0.125;
             s_r[rowX + iv][t] = dtdx * sgX * Fin[iv] * 0.125;
         }
+        if constexpr (!RANKED) {
+            s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
+            s_k[slotX][t] = Kin;
+        }
     }
     if (!leaf) return;
-    ordered_apply_ranked<O_BLOCK>(m, cC, fs, dtdx, gp, s_r, t, nbefore, nvalid);
+    if constexpr (RANKED) ordered_apply_ranked<O_BLOCK>(m, cC, fs, dtdx, gp, s_r, t, nbefore, nvalid);
+    else ordered_apply<O_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t);
 }
 
 // Reflux into coarser leaf cells, one thread per target cell, contributions

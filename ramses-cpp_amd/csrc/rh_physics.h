@@ -208,26 +208,6 @@ RH_DEV void ghost_slopes(const T a[5], const T b[5], const T c[5], int interpol_
     for (int iv = 0; iv < 5; ++iv) dq[iv] = over(ghost_limiter(b[iv] - a[iv], c[iv] - b[iv], interpol_type), lc);
 }
 
-// Both face-state pairs of one cell from rotated primitives: its own states
-// (trace pass, HydroSolver.cpp:93-115: slope_type slopes over dx) and its
-// ghost states as a coarser neighbour's (HydroSolver.cpp:357-440:
-// interpol_type slopes over 2*dx).  The two traces share the cell's floors
-// and sound speed.
-template <typename T, bool E>
-RH_DEV void trace_pair(const T a[5], const T b[5], const T c[5], int slope_type, int interpol_type, T dt,
-                       const Len<T, E>& len, T om[5], T op[5], T gm[5], T gp[5], const Gas<T>& g) {
-    T dqo[5], dqg[5];
-    own_slopes(a, b, c, slope_type, dt, len, dqo);
-    const Len<T, E> lc = coarse(len);
-    ghost_slopes(a, b, c, interpol_type, lc, dqg);
-    const T r = smax(b[0], T(1e-10));
-    const T p = smax(b[4], r * T(1e-20));
-    const T c2 = cs2(r, p, g);
-    const T hdt = T(0.5) * dt;
-    trace_core(b, dqo, r, b[1], c2, T(0.5) * len.dx, hdt, om, op);
-    trace_core(b, dqg, r, b[1], c2, T(0.5) * lc.dx, hdt, gm, gp);
-}
-
 // Ghost face states only (a coarser neighbour's), rotated primitives.
 template <typename T, bool E>
 RH_DEV void ghost_trace_rot(const T a[5], const T b[5], const T c[5], int interpol_type, T dt, const Len<T, E>& len,
