@@ -156,8 +156,10 @@ def measure_amr(args, rhydro, np, which):
     L.rh_timers_reset(ctx)
     _, _, cu0 = run.stats()
     t0 = time.perf_counter()
+    ph0 = run.phase_ms()
     run.step(steps)
     wall = time.perf_counter() - t0
+    ph1 = run.phase_ms()
     t_sim, nstep, cu1 = run.stats()
     ms, n = C_double(), C_longlong()
     L.rh_timers_read(ctx, rhydro.TK_GODUNOV, byref(ms), byref(n))
@@ -165,7 +167,8 @@ def measure_amr(args, rhydro, np, which):
     run.close()
     out = {"workload": name, "coarse_steps": steps, "fine_steps": nstep, "cell_updates": int(cu1 - cu0),
            "value": (cu1 - cu0) / wall, "wall_s": wall, "init_s": t_init, "godunov_ms_total": ms.value,
-           "godunov_launches": n.value, "octs_per_level": st["numbl"].tolist()}
+           "godunov_launches": n.value, "octs_per_level": st["numbl"].tolist(),
+           "host_phase_ms": {k: ph1[k] - ph0[k] for k in ph1}}
     print(f"[bench] amr {which}: {steps} coarse steps, {cu1 - cu0} cell-updates in {wall:.3f}s "
           f"(godunov {ms.value:.1f} ms), lists {st['numbl'].tolist()}", file=sys.stderr, flush=True)
     if not args.no_cpu and which == "sedov":

@@ -22,10 +22,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rh_context.h"
@@ -64,6 +66,9 @@ struct rh_amr {
     long long epoch = 1;
     rh_mesh mesh{};
     std::string err;
+    // Host wall time per phase of the coarse steps (ms): hydro calls, flag_fine,
+    // make_grid_fine, remove_grid_fine, restrict_fine.
+    double phase_ms[5] = {0, 0, 0, 0, 0};
 };
 
 using namespace rhi;
@@ -89,6 +94,17 @@ int afail(rh_amr* a, int code, const std::string& msg) {
     } while (0)
 
 int cell(const rh_amr* a, int igrid, int ic) { return a->ncoarse + (ic - 1) * a->ngridmax + igrid; }
+
+enum { PH_HYDRO = 0, PH_FLAG = 1, PH_MAKE = 2, PH_REMOVE = 3, PH_RESTRICT = 4 };
+
+struct Phase {
+    rh_amr* a;
+    int k;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    ~Phase() {
+        a->phase_ms[k] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
 
 void refresh_mesh(rh_amr* a) {
     rh_mesh& m = a->mesh;
@@ -312,6 +328,39 @@ void nbor_exact(const rh_amr* a, int ig, int ic, int icn[6]) {
         }
 }
 
+// ---- host parallelism ------------------------------------------------------
+
+// Runs fn(i) for i in [0, n) on the host's cores (the flag passes touch each
+// oct's own cells only, or defer their writes, so octs are independent).
+template <class Fn> void parallel_for(int n, Fn fn) {
+    static const int nthreads = [] {
+        unsigned h = std::thread::hardware_concurrency();
+        return (int)std::max(1u, std::min(h == 0 ? 1u : h, 16u));
+    }();
+    if (n < 4096 || nthreads == 1) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int chunk = (n + nthreads - 1) / nthreads;
+    for (int k = 0; k < nthreads; ++k) {
+        const int lo = k * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) break;
+        pool.emplace_back([lo, hi, &fn] {
+            for (int i = lo; i < hi; ++i) fn(i);
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
+// The grids of a level list, in list order.
+std::vector<int> list_grids(const rh_amr* a, int il) {
+    std::vector<int> g;
+    g.reserve(a->numbl[il]);
+    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1]) g.push_back(ig);
+    return g;
+}
+
 // ---- tree updater (TreeUpdater.cpp) ---------------------------------------
 
 bool child_active(const rh_amr* a, int idc0) {
@@ -325,9 +374,8 @@ bool child_active(const rh_amr* a, int idc0) {
 }
 
 void smooth_fine(rh_amr* a, int il) {
-    std::vector<int> ok;
     if (il == 1) {
-        ok.assign(a->ncoarse, 0);
+        std::vector<int> ok(a->ncoarse, 0);
         for (int ic = 1; ic <= a->ncoarse; ++ic) {
             if (a->flag1[ic - 1] != 0) continue;
             int icn[6];
@@ -341,19 +389,26 @@ void smooth_fine(rh_amr* a, int il) {
             if (ok[i]) a->flag1[i] = 1;
         return;
     }
-    std::vector<int> set;
-    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1])
+    const std::vector<int> grids = list_grids(a, il);
+    std::vector<unsigned char> ok(grids.size(), 0);  // bit ic-1 of each oct
+    parallel_for((int)grids.size(), [&](int k) {
+        const int ig = grids[k];
+        unsigned char bits = 0;
         for (int ic = 1; ic <= 8; ++ic) {
             int idc = cell(a, ig, ic) - 1;
             if (a->flag1[idc] != 0) continue;
             int icn[6];
             nbor_exact(a, ig, ic, icn);
             int nf = 0;
-            for (int k = 0; k < 6; ++k)
-                if (icn[k] > 0 && icn[k] <= a->ncell && a->flag1[icn[k] - 1] == 1) nf++;
-            if (nf > 0) set.push_back(idc);
+            for (int f = 0; f < 6; ++f)
+                if (icn[f] > 0 && icn[f] <= a->ncell && a->flag1[icn[f] - 1] == 1) nf++;
+            if (nf > 0) bits |= (unsigned char)(1u << (ic - 1));
         }
-    for (int idc : set) a->flag1[idc] = 1;
+        ok[k] = bits;
+    });
+    for (size_t k = 0; k < grids.size(); ++k)
+        for (int ic = 1; ic <= 8; ++ic)
+            if ((ok[k] >> (ic - 1)) & 1) a->flag1[cell(a, grids[k], ic) - 1] = 1;
 }
 
 void test_flag(rh_amr* a, int il) {
@@ -362,20 +417,26 @@ void test_flag(rh_amr* a, int il) {
             if (child_active(a, ic - 1)) a->flag1[ic - 1] = 1;
         return;
     }
-    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1])
+    // Each cell's test reads only its own children, which are not flagged
+    // here, so the octs are independent.
+    const std::vector<int> grids = list_grids(a, il);
+    parallel_for((int)grids.size(), [&](int k) {
         for (int ic = 1; ic <= 8; ++ic) {
-            int idc = cell(a, ig, ic) - 1;
+            int idc = cell(a, grids[k], ic) - 1;
             if (child_active(a, idc)) a->flag1[idc] = 1;
         }
+    });
 }
 
 void ensure_ref_rules(rh_amr* a, int il) {
-    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1]) {
+    const std::vector<int> grids = list_grids(a, il);
+    parallel_for((int)grids.size(), [&](int k) {
+        const int ig = grids[k];
         int fc = a->father[ig - 1];
         bool ok = fc > 0 && neighbours_refined(a, fc);
         if (!ok)
             for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
-    }
+    });
 }
 
 // Gradient criteria of the coarse cells (TreeUpdater.cpp:459-480), on the
@@ -454,12 +515,13 @@ int list_gradients(rh_amr* a, int il) {
     std::vector<unsigned char> fl((size_t)lv->noct * 8);
     RH_AHIP(a, hipMemcpyAsync(fl.data(), a->d_flags, fl.size(), hipMemcpyDeviceToHost, a->ctx->stream));
     RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
-    int pos = 0;
-    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1], ++pos)
+    const std::vector<int> grids = list_grids(a, il);
+    parallel_for((int)grids.size(), [&](int pos) {
         for (int ic = 1; ic <= 8; ++ic) {
-            int idc = cell(a, ig, ic) - 1;
+            int idc = cell(a, grids[pos], ic) - 1;
             if (a->flag1[idc] == 0 && fl[(size_t)pos * 8 + ic - 1]) a->flag1[idc] = 1;
         }
+    });
     return RH_OK;
 }
 
@@ -472,11 +534,13 @@ int flag_fine(rh_amr* a, int il, int icount, int nsub) {
         for (int i = 1; i <= a->ncoarse; ++i)
             if (il < lmin || child_active(a, i - 1)) a->flag1[i - 1] = 1;
     } else {
-        for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1])
+        const std::vector<int> grids = list_grids(a, il);
+        parallel_for((int)grids.size(), [&](int k) {
             for (int ic = 1; ic <= 8; ++ic) {
-                int idc = cell(a, ig, ic) - 1;
+                int idc = cell(a, grids[k], ic) - 1;
                 a->flag1[idc] = (il < lmin || child_active(a, idc)) ? 1 : 0;
             }
+        });
     }
     if (il >= lmin) smooth_fine(a, il);
     if (il >= lmin && (a->ap.err_grad_d > 0.0 || a->ap.err_grad_p > 0.0 || a->ap.err_grad_v > 0.0))
@@ -679,17 +743,27 @@ int amr_step(rh_amr* a, int il, int icount) {
     if (il > a->nlevelmax) return RH_OK;
     if (count(a, il) == 0 && il > 0) return RH_OK;
     if (a->levelmin < a->nlevelmax && (il == a->levelmin || icount > 1)) {
-        for (int i = il; i <= a->nlevelmax; ++i) RH_ACHK(make_grid_fine(a, i + 1));
+        {
+            Phase ph{a, PH_MAKE};
+            for (int i = il; i <= a->nlevelmax; ++i) RH_ACHK(make_grid_fine(a, i + 1));
+        }
+        Phase ph{a, PH_REMOVE};
         for (int i = a->nlevelmax; i >= il; --i) remove_grid_fine(a, i + 1);
     }
     const double dx = a->boxlen / (double)(a->nx * (1 << il));
     a->dtold[il] = a->dtnew[il];
     double dtc = 0.0;
-    RH_ACHK(rh_courant(a->ctx, il, dx, a->gamma, a->ap.courant_factor, &dtc));
+    {
+        Phase ph{a, PH_HYDRO};
+        RH_ACHK(rh_courant(a->ctx, il, dx, a->gamma, a->ap.courant_factor, &dtc));
+    }
     a->dtnew[il] = dtc;
     if (il > a->levelmin) a->dtnew[il] = std::min(a->dtnew[il], a->dtnew[il - 1] / (double)a->nsubcycle[il - 1]);
     const double dt = a->dtnew[il];
-    RH_ACHK(rh_set_unew(a->ctx, il));
+    {
+        Phase ph{a, PH_HYDRO};
+        RH_ACHK(rh_set_unew(a->ctx, il));
+    }
     const int nsub = il < (int)a->nsubcycle.size() ? a->nsubcycle[il] : 1;
     if (il < a->nlevelmax) {
         if (count(a, il + 1) > 0) {
@@ -704,22 +778,36 @@ int amr_step(rh_amr* a, int il, int icount) {
         a->t += dt;
         a->nstep++;
     }
-    long long nleaf = 0;
-    RH_ACHK(rh_level_leaf_cells(a->ctx, il, &nleaf));
-    a->cell_updates += nleaf;
-    RH_ACHK(rh_godunov_fine(a->ctx, il, dt, dx));
-    RH_ACHK(rh_set_uold(a->ctx, il));
-    if (il < a->nlevelmax) RH_ACHK(restrict_fine(a, il));
+    {
+        Phase ph{a, PH_HYDRO};
+        long long nleaf = 0;
+        RH_ACHK(rh_level_leaf_cells(a->ctx, il, &nleaf));
+        a->cell_updates += nleaf;
+        RH_ACHK(rh_godunov_fine(a->ctx, il, dt, dx));
+        RH_ACHK(rh_set_uold(a->ctx, il));
+    }
+    if (il < a->nlevelmax) {
+        Phase ph{a, PH_RESTRICT};
+        RH_ACHK(restrict_fine(a, il));
+    }
+    Phase ph{a, PH_FLAG};
     return flag_fine(a, il + 1, icount, il > 0 ? a->nsubcycle[il - 1] : 1);
 }
 
 int step_coarse(rh_amr* a) {
-    if (a->levelmin < a->nlevelmax)
+    if (a->levelmin < a->nlevelmax) {
+        Phase ph{a, PH_MAKE};
         for (int il = 1; il <= a->levelmin - 1; ++il) RH_ACHK(make_grid_fine(a, il));
+    }
     RH_ACHK(amr_step(a, a->levelmin, 1));
-    for (int il = a->levelmin; il >= 1; --il) RH_ACHK(flag_fine(a, il, 2, 2));
-    if (a->levelmin < a->nlevelmax)
+    {
+        Phase ph{a, PH_FLAG};
+        for (int il = a->levelmin; il >= 1; --il) RH_ACHK(flag_fine(a, il, 2, 2));
+    }
+    if (a->levelmin < a->nlevelmax) {
+        Phase ph{a, PH_RESTRICT};
         for (int il = a->levelmin - 1; il >= 0; --il) RH_ACHK(restrict_fine(a, il));
+    }
     return RH_OK;
 }
 
@@ -821,5 +909,12 @@ int rh_amr_stats(rh_amr* a, double* t, int* nstep, long long* cell_updates) {
 }
 
 rh_ctx* rh_amr_context(rh_amr* a) { return a ? a->ctx : nullptr; }
+
+int rh_amr_timers(rh_amr* a, double* ms) {
+    if (!a || !ms) return RH_ERR_ARG;
+    for (int k = 0; k < 5; ++k) ms[k] = a->phase_ms[k];
+    ms[5] = a->ctx->table_ms;
+    return RH_OK;
+}
 
 }  // extern "C"

@@ -3,6 +3,7 @@
 // rh_hydro.hip.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -73,6 +74,13 @@ rh::GasParams gas(const rh_ctx* c, double gamma) {
 // Level tables for ilevel, rebuilt when the mesh epoch moved.
 int get_level(rh_ctx* c, int ilevel, const rh::LevelDev** out) {
     if (ilevel < 1 || ilevel > c->mesh.nlevelmax) return fail(c, RH_ERR_ARG, "ilevel out of range");
+    struct Clock {
+        rh_ctx* c;
+        std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+        ~Clock() {
+            c->table_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+    } clock{c};
     if (!c->topo_valid || c->topo.epoch != c->mesh.mesh_epoch) {
         std::string e;
         int st = rh::build_topo_index(c->mesh, c->topo, e);

@@ -58,6 +58,7 @@ struct rh_ctx {
     int variant = 1;  // Godunov kernel variant (rh_set_kernel_variant)
     bool timers = false;
     double t_ms[rhi::TK_N] = {0, 0, 0, 0};
+    double table_ms = 0.0;  // host time spent rebuilding level tables
     long long t_n[rhi::TK_N] = {0, 0, 0, 0};
     // Event pairs recorded around launches, summed when the timers are read.
     std::vector<hipEvent_t> ev_pool;

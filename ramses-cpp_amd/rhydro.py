@@ -119,6 +119,7 @@ def lib() -> C.CDLL:
         L.rh_amr_stats.argtypes = [C.c_void_p, _DP, _IP, C.POINTER(C.c_longlong)]
         L.rh_amr_context.argtypes = [C.c_void_p]
         L.rh_amr_context.restype = C.c_void_p
+        L.rh_amr_timers.argtypes = [C.c_void_p, _DP]
         L.rh_init_shock_tube.argtypes = [C.POINTER(RhMesh), _DP, C.c_double, C.c_double, _DP, _DP, _DP, _DP]
         L.rh_bind_ensemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhMesh), C.c_int]
         L.rh_run_ensemble.argtypes = [C.c_void_p, _IP, _DP, C.c_double, C.c_int, _DP]
@@ -134,7 +135,7 @@ EXPORTED = [
     "rh_level_leaf_cells", "rh_set_kernel_variant", "rh_get_kernel_variant", "rh_autotune", "rh_init_shock_tube",
     "rh_bind_ensemble", "rh_run_ensemble", "rh_ensemble_leaf_cells", "rh_add_gravity_source_terms",
     "rh_synchro_hydro_fine", "rh_amr_create", "rh_amr_destroy", "rh_amr_step", "rh_amr_mesh", "rh_amr_stats",
-    "rh_amr_context",
+    "rh_amr_context", "rh_amr_timers",
 ]
 
 
@@ -532,6 +533,11 @@ class AmrRun:
                     son=arr(m.son, nc), nbor=arr(m.nbor, 6 * ng), next=arr(m.next, ng), headl=arr(m.headl, nl + 1),
                     numbl=arr(nb, nl + 1), xg=arr(xg, 3 * ng), uold=arr(m.uold, 5 * nc), unew=arr(m.unew, 5 * nc),
                     t=t, nstep=nstep, cell_updates=cu)
+
+    def phase_ms(self) -> dict:
+        ms = np.zeros(6, np.float64)
+        self._chk(lib().rh_amr_timers(self._h, _dp(ms)))
+        return dict(zip(("hydro", "flag", "make", "remove", "restrict", "tables"), ms.tolist()))
 
     def solver_context(self):
         return lib().rh_amr_context(self._h)
