@@ -513,7 +513,7 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
     // (HydroSolver.cpp:130-145).
     auto boundary = [&](int d, TC qm[5], TC qp[5]) {
         double u[5], qd[5];
-        copy5(uC, u);
+        load_u(m.uold, m.ncell, cC, u);
 #pragma unroll
         for (int i = 1; i <= 3; ++i)
             if (i == 1 + d) u[i] = u[i] * -1.0;
@@ -668,17 +668,15 @@ template <typename T> RH_DEV void sel5(bool c, const T a[5], const T b[5], T out
 constexpr int O_OCTS = 16;
 constexpr int O_BLOCK = O_OCTS * 8;
 
-// Occupancy: the mixed form with HLLC or the acoustic solver fits three
-// waves per SIMD in 168 VGPRs (a few spilled in cold paths) once its LDS
-// payload drops the keys: contributions are written in processing order
-// (RANKED) and 30 KB of LDS per workgroup lets five workgroups share a CU
-// (measured 3.5% faster at 256^3 for HLLC).  With LLF or HLL the same cap
-// spills in the hot path and runs slower, and the FP64 form is register-bound
-// at two waves either way; those keep the keys beside the contributions and
-// sort them at the end.
+// Occupancy: the mixed form fits three waves per SIMD in 168 VGPRs once its
+// LDS payload drops the keys: contributions are written in processing order
+// (RANKED) and 30 KB of LDS per workgroup lets five workgroups share a CU.
+// The mixed HLL solver would spill at that cap and stays at two waves; the
+// FP64 form is register-bound at two waves either way.  Those keep the keys
+// beside the contributions and sort them at the end (faster for them).
 template <int R, bool M>
 constexpr bool oct_ranked() {
-    return M && (R == 2 || R == 4);
+    return M && R != 1;
 }
 
 template <int RIEMANN, bool MIXED, bool EXACT>
@@ -822,8 +820,10 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
             }
         } else if (rc.w & REC_REFINED_DEV) {
         } else if (rc.w & REC_BOUNDARY_DEV) {
+            // The cell's own state, read again rather than kept in registers
+            // through the loop for this rare branch.
             double u[5], qd[5];
-            copy5(uC, u);
+            load_u(m.uold, m.ncell, cC, u);
 #pragma unroll
             for (int i = 1; i <= 3; ++i)
                 if (i == 1 + d) u[i] = u[i] * -1.0;
@@ -890,7 +890,10 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
 }
 
 // Reflux into coarser leaf cells, one thread per target cell, contributions
-// in processing order.
+// in processing order.  The slot indices and values of up to eight
+// contributions are loaded together before they are subtracted in order, so
+// a target with many contributions waits for a few memory round trips, not
+// one per contribution.
 __global__ __launch_bounds__(256) void k_coarse_reflux(MeshDev m, LevelDev lv) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= lv.ntgt) return;
@@ -898,10 +901,22 @@ __global__ __launch_bounds__(256) void k_coarse_reflux(MeshDev m, LevelDev lv) {
     double u[5];
 #pragma unroll
     for (int iv = 0; iv < 5; ++iv) u[iv] = m.unew[(size_t)iv * m.ncell + c];
-    for (int k = lv.tgt_off[t]; k < lv.tgt_off[t + 1]; ++k) {
-        const size_t s = (size_t)lv.tgt_slot[k] * 5;
+    const int end = lv.tgt_off[t + 1];
+    for (int k = lv.tgt_off[t]; k < end; k += 8) {
+        const int n = min(8, end - k);
+        int idx[8];
 #pragma unroll
-        for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] - lv.slot_r[s + iv];
+        for (int j = 0; j < 8; ++j) idx[j] = j < n ? lv.tgt_slot[k + j] : 0;
+        double v[8][5];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) v[j][iv] = j < n ? lv.slot_r[(size_t)idx[j] * 5 + iv] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j < n)
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] - v[j][iv];
     }
 #pragma unroll
     for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
