@@ -254,6 +254,8 @@ def main() -> int:
     ap.add_argument("--boxes", type=int, default=96, help="boxes of the shock-tube ensemble (0: skip it)")
     ap.add_argument("--no-variants", action="store_true", help="skip the HLL / slope-2 variants of the workload")
     ap.add_argument("--no-sedov", action="store_true", help="skip the 64^3 Sedov run")
+    ap.add_argument("--only", choices=["ensemble", "amr_sedov", "amr_kh", "sedov64"],
+                    help="run one secondary workload alone and print its record (for profiling)")
     ap.add_argument("--amr-steps", type=int, default=10, help="coarse steps of the AMR Sedov run (0: skip)")
     ap.add_argument("--kh-steps", type=int, default=1, help="coarse steps of the AMR Kelvin-Helmholtz run (0: skip)")
     args = ap.parse_args()
@@ -266,6 +268,16 @@ def main() -> int:
     import numpy as np
 
     import rhydro
+
+    if args.only:
+        if args.only == "ensemble":
+            rec = measure_ensemble(args, rhydro, np)
+        elif args.only == "sedov64":
+            rec = measure_sedov64(args, rhydro, np)
+        else:
+            rec = measure_amr(args, rhydro, np, "sedov" if args.only == "amr_sedov" else "kh")
+        print(json.dumps({"only": args.only, "record": rec}))
+        return 0
 
     gamma = 5.0 / 3.0
     level = args.level
