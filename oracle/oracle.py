@@ -338,6 +338,12 @@ class OracleAmr:
                     xg=dbl(s.xg, 3 * ng), uold=dbl(s.uold, 5 * nc), unew=dbl(s.unew, 5 * nc), t=s.t,
                     nstep=s.nstep, dtnew=dbl(s.dtnew, 33), cell_updates=s.cell_updates, status=s.status)
 
+    def dump(self, root: str, iout: int, **info) -> None:
+        """dump_snapshot of the run into <root>/output_<iout> (rh_oracle_snapshot.cpp)."""
+        st = _snap_lib().or_amr_dump(self._h, C.byref(snapshot_info(**info)), iout, os.fsencode(root))
+        if st != 0:
+            raise OSError(f"oracle snapshot into {root} failed")
+
     def close(self):
         if self._h:
             _amr_lib().or_amr_destroy(self._h)
@@ -348,3 +354,81 @@ class OracleAmr:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# Snapshot writer (rh_oracle_snapshot.cpp): RamsesWriter's files, restated.
+
+class OrSnapshotInfo(C.Structure):
+    _fields_ = [("t", C.c_double)] + [(k, C.c_int) for k in ("nstep", "nstep_coarse", "noutput", "iout", "nener")] + [
+        (k, C.c_double) for k in ("einit", "mass_tot_0", "rho_tot", "omega_m", "omega_l", "omega_k", "omega_b",
+                                  "h0", "aexp_ini", "boxlen_ini", "aexp", "hexp", "aexp_old", "epot_tot_int",
+                                  "epot_tot_old", "mass_sph")] + [
+        ("levelmin", C.c_int), ("levelmax", C.c_int)] + [
+        (k, C.c_double) for k in ("boxlen", "units_length", "units_density", "units_time")]
+
+
+class OrSnapshotTree(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("nx", "ny", "nz", "ncoarse", "ngridmax", "ncell", "nlevelmax", "ncpu",
+                                       "nboundary")] + [
+        ("gamma", C.c_double), ("headl", _IP), ("taill", _IP), ("numbl", _IP),
+        ("headf", C.c_int), ("tailf", C.c_int), ("numbf", C.c_int)] + [
+        (k, _IP) for k in ("next", "prev", "father", "nbor", "son", "flag1", "cpu_map")] + [
+        ("xg", _DP), ("phi", _DP), ("f", _DP), ("npart", C.c_longlong)]
+
+
+def snapshot_info(**kw) -> OrSnapshotInfo:
+    """SnapshotInfo's defaults (RamsesWriter.hpp:24-31; aexp 1, units 1),
+    overridden by keyword."""
+    i = OrSnapshotInfo()
+    i.aexp_ini = i.boxlen_ini = i.aexp = i.aexp_old = 1.0
+    i.units_length = i.units_density = i.units_time = 1.0
+    i.boxlen = 1.0
+    for k, v in kw.items():
+        setattr(i, k, v)
+    return i
+
+
+def _snap_lib():
+    L = lib()
+    if not getattr(L, "_snap_ready", False):
+        L.or_dump_snapshot.argtypes = [C.POINTER(OrSnapshotTree), _DP, C.POINTER(OrSnapshotInfo), C.c_char_p]
+        L.or_amr_dump.argtypes = [C.c_void_p, C.POINTER(OrSnapshotInfo), C.c_int, C.c_char_p]
+        L._snap_ready = True
+    return L
+
+
+def dump_snapshot(arrays: dict, dims: dict, root: str, **info) -> None:
+    """dump_snapshot from host arrays: `arrays` holds the AmrGrid arrays by
+    name (headl, taill, numbl, next, prev, father, nbor, son, xg, uold; flag1,
+    cpu_map, phi, f optional), `dims` nx, ny, nz, ngridmax, nlevelmax, gamma
+    and optionally headf, tailf, numbf, nboundary, npart."""
+    t = OrSnapshotTree()
+    t.nx, t.ny, t.nz = dims["nx"], dims["ny"], dims["nz"]
+    t.ncoarse = t.nx * t.ny * t.nz
+    t.ngridmax, t.nlevelmax = dims["ngridmax"], dims["nlevelmax"]
+    t.ncell = t.ncoarse + 8 * t.ngridmax
+    t.ncpu, t.nboundary = 1, dims.get("nboundary", 0)
+    t.gamma = dims["gamma"]
+    t.headf, t.tailf, t.numbf = dims.get("headf", 0), dims.get("tailf", 0), dims.get("numbf", 0)
+    t.npart = dims.get("npart", 0)
+    keep = []
+    for k in ("headl", "taill", "numbl", "next", "prev", "father", "nbor", "son", "flag1", "cpu_map"):
+        a = arrays.get(k)
+        if a is None:
+            setattr(t, k, C.cast(None, _IP))
+        else:
+            a = np.ascontiguousarray(a, np.int32)
+            keep.append(a)
+            setattr(t, k, _ip(a))
+    for k in ("xg", "phi", "f"):
+        a = arrays.get(k)
+        if a is None:
+            setattr(t, k, C.cast(None, _DP))
+        else:
+            a = np.ascontiguousarray(a, np.float64)
+            keep.append(a)
+            setattr(t, k, _dp(a))
+    u = np.ascontiguousarray(arrays["uold"], np.float64)
+    if _snap_lib().or_dump_snapshot(C.byref(t), _dp(u), C.byref(snapshot_info(**info)), os.fsencode(root)) != 0:
+        raise OSError(f"oracle snapshot into {root} failed")

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "rh_oracle_core.h"
+#include "rh_oracle_snapshot.h"
 
 extern "C" {
 
@@ -878,5 +879,44 @@ void or_amr_get(void* h, or_amr_state* s) {
 }
 
 void or_amr_destroy(void* h) { delete (Amr*)h; }
+
+// dump_snapshot of the run (Simulation.cpp:678-732): t, nstep, levelmin,
+// levelmax and boxlen from the run, the rest from *info; gamma from the run.
+int or_amr_dump(void* h, const or_snapshot_info* info, int iout, const char* root) {
+    Amr* a = (Amr*)h;
+    or_snapshot_info in = *info;
+    in.t = a->t;
+    in.nstep = in.nstep_coarse = a->nstep;
+    in.iout = iout;
+    in.levelmin = a->levelmin;
+    in.levelmax = a->nlevelmax;
+    in.boxlen = a->boxlen;
+    or_snapshot_tree t{};
+    t.nx = a->nx;
+    t.ny = a->ny;
+    t.nz = a->nz;
+    t.ncoarse = a->ncoarse;
+    t.ngridmax = a->ngridmax;
+    t.ncell = a->ncell;
+    t.nlevelmax = a->nlevelmax;
+    t.ncpu = 1;
+    t.nboundary = 0;
+    t.gamma = a->gamma;
+    t.headl = a->headl.data();
+    t.taill = a->taill.data();
+    t.numbl = a->numbl.data();
+    t.headf = a->headf;
+    t.tailf = a->tailf;
+    t.numbf = a->numbf;
+    t.next = a->next.data();
+    t.prev = a->prev.data();
+    t.father = a->father.data();
+    t.nbor = a->nbor.data();
+    t.son = a->son.data();
+    t.flag1 = a->flag1.data();
+    t.cpu_map = a->cpu_map.data();
+    t.xg = a->xg.data();
+    return or_dump_snapshot(&t, a->uold.data(), &in, root);
+}
 
 }  // extern "C"

@@ -36,6 +36,8 @@
 #include "rhydro.h"
 #include "rhydro_amr.h"
 #include "rhydro_driver.h"
+#include "rhydro_snapshot.h"
+#include "rh_snapshot.h"
 
 struct rh_amr {
     rh_ctx* ctx = nullptr;
@@ -909,6 +911,50 @@ int rh_amr_stats(rh_amr* a, double* t, int* nstep, long long* cell_updates) {
 }
 
 rh_ctx* rh_amr_context(rh_amr* a) { return a ? a->ctx : nullptr; }
+
+int rh_amr_dump(rh_amr* a, const rh_snapshot_info* info, int iout, const char* root) {
+    if (!a || !root) return RH_ERR_ARG;
+    rh_snapshot_info in{};
+    if (info) {
+        in = *info;
+    } else {  // SnapshotInfo's defaults (RamsesWriter.hpp:24-31), no outputs, unit scales 1
+        in.aexp_ini = in.boxlen_ini = in.aexp = in.aexp_old = 1.0;
+        in.units_length = in.units_density = in.units_time = 1.0;
+    }
+    // What Simulation::dump_snapshot takes from the run (Simulation.cpp:683-692).
+    in.t = a->t;
+    in.nstep = in.nstep_coarse = a->nstep;
+    in.iout = iout;
+    in.levelmin = a->levelmin;
+    in.levelmax = a->nlevelmax;
+    in.boxlen = a->boxlen;
+    rh_snapshot_tree t{};
+    t.nx = a->nx;
+    t.ny = a->ny;
+    t.nz = a->nz;
+    t.ncoarse = a->ncoarse;
+    t.ngridmax = a->ngridmax;
+    t.ncell = a->ncell;
+    t.nlevelmax = a->nlevelmax;
+    t.ncpu = 1;
+    t.nboundary = 0;
+    t.gamma = a->gamma;
+    t.headl = a->headl.data();
+    t.taill = a->taill.data();
+    t.numbl = a->numbl.data();
+    t.headf = a->headf;
+    t.tailf = a->tailf;
+    t.numbf = a->numbf;
+    t.next = a->next.data();
+    t.prev = a->prev.data();
+    t.father = a->father.data();
+    t.nbor = a->nbor.data();
+    t.son = a->son.data();
+    t.flag1 = a->flag1.data();
+    t.cpu_map = nullptr;  // every cell belongs to rank 1
+    t.xg = a->xg.data();
+    return rhi::dump_snapshot(a->ctx, &t, &in, root);
+}
 
 int rh_amr_timers(rh_amr* a, double* ms) {
     if (!a || !ms) return RH_ERR_ARG;

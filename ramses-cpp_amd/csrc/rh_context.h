@@ -93,6 +93,10 @@ int ensure_dt(rh_ctx* c, int n);
 // Level tables of ilevel for the bound mesh, rebuilt when the epoch moved.
 int get_level(rh_ctx* c, int ilevel, const rh::LevelDev** out);
 void collect_timers(rh_ctx* c);
+// A single mesh is bound (not an ensemble); else an error on c.
+int check_bound(rh_ctx* c);
+// Host-coherent mode: uploads the host state (and f) before a call.
+int pre_call(rh_ctx* c);
 // Frees every device buffer of the current binding (single or ensemble).
 void unbind(rh_ctx* c);
 

@@ -1137,6 +1137,59 @@ __global__ __launch_bounds__(256) void k_grad_flags(MeshDev m, LevelDev lv, cons
     out[k] = ok ? 1 : 0;
 }
 
+// One level of a snapshot's hydro file (RamsesWriter.cpp:163-233), as the
+// 32-bit words of its Fortran records: the il and ncache records, then (when
+// n > 0) 40 records of n primitive values -- cell position ic outer,
+// variable inner -- each framed by its byte count, then nbnd pairs of (il, 0)
+// records for the boundary domains.  Thread i packs grid i of the level list
+// (grids[i]); threads 0..39 also write the frames, thread 0 the small
+// records.  A double sits at an odd word index, so it is stored as two words.
+__global__ __launch_bounds__(256) void k_pack_hydro(MeshDev m, const int* __restrict__ grids, int n, int il,
+                                                    int nbnd, double gamma, uint32_t* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const size_t rw = 2 + 2 * (size_t)n;  // words per value record
+    const size_t body = n > 0 ? 40 * rw : 0;
+    if (i == 0) {
+        const uint32_t head[6] = {4u, (uint32_t)il, 4u, 4u, (uint32_t)n, 4u};
+        for (int w = 0; w < 6; ++w) out[w] = head[w];
+        for (int b = 0; b < nbnd; ++b) {
+            const uint32_t tail[6] = {4u, (uint32_t)il, 4u, 4u, 0u, 4u};
+            for (int w = 0; w < 6; ++w) out[6 + body + 6 * (size_t)b + w] = tail[w];
+        }
+    }
+    if (n > 0 && i < 40) {
+        out[6 + i * rw] = 8u * (uint32_t)n;
+        out[6 + i * rw + rw - 1] = 8u * (uint32_t)n;
+    }
+    if (i >= n) return;
+    const int g = grids[i];
+    const double smallr = 1e-10;
+    for (int ic0 = 0; ic0 < 8; ++ic0) {
+        const size_t c = (size_t)cell_id(m, g, ic0) - 1;
+        double u[5];
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) u[iv] = m.uold[(size_t)iv * m.ncell + c];
+        const double d = u[0] < smallr ? smallr : u[0];
+        double q[5];
+        q[0] = d;
+        double v2 = 0.0;
+#pragma unroll
+        for (int j = 1; j <= 3; ++j) {
+            q[j] = u[j] / d;
+            v2 += q[j] * q[j];
+        }
+        const double p = (u[4] - 0.5 * d * v2 - 0.0) * (gamma - 1.0);
+        q[4] = p < 0.0 ? 0.0 : p;
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(q[iv]);
+            const size_t w = 6 + (size_t)(ic0 * 5 + iv) * rw + 1 + 2 * (size_t)i;
+            out[w] = (uint32_t)bits;
+            out[w + 1] = (uint32_t)(bits >> 32);
+        }
+    }
+}
+
 inline int blocks_for(int noct) { return (noct + OCTS_PER_BLOCK - 1) / OCTS_PER_BLOCK; }
 
 }  // namespace
@@ -1271,6 +1324,16 @@ hipError_t launch_interpol(int n_octs, const double* d_u1, double* d_u2, const G
     if (n_octs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_interpol, dim3((n_octs + 63) / 64), dim3(64), 0, stream, n_octs, d_u1, d_u2, gp,
                        interpol_var);
+    return hipGetLastError();
+}
+
+size_t pack_hydro_words(int n, int nbnd) { return 6 + (n > 0 ? 40 * (2 + 2 * (size_t)n) : 0) + 6 * (size_t)nbnd; }
+
+hipError_t launch_pack_hydro(const MeshDev& m, const int* d_grids, int n, int il, int nbnd, double gamma,
+                             uint32_t* d_out, hipStream_t stream) {
+    const int threads = n > 40 ? n : 40;
+    hipLaunchKernelGGL(k_pack_hydro, dim3((threads + 255) / 256), dim3(256), 0, stream, m, d_grids, n, il, nbnd,
+                       gamma, d_out);
     return hipGetLastError();
 }
 

@@ -89,5 +89,10 @@ hipError_t launch_grad_flags(const MeshDev& m, const LevelDev& lv, const int* d_
                              double ev, double gamma, unsigned char* d_out, hipStream_t stream);
 hipError_t launch_interpol(int n_octs, const double* d_u1, double* d_u2, const GasParams& gp, int interpol_var,
                            hipStream_t stream);
+// Snapshot hydro records of one level list (n grids, d_grids in list order)
+// as 32-bit words, pack_hydro_words(n, nbnd) of them, into d_out.
+size_t pack_hydro_words(int n, int nbnd);
+hipError_t launch_pack_hydro(const MeshDev& m, const int* d_grids, int n, int il, int nbnd, double gamma,
+                             uint32_t* d_out, hipStream_t stream);
 
 }  // namespace rh

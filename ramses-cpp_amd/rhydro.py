@@ -54,6 +54,40 @@ class RhMesh(C.Structure):
     ]
 
 
+class RhSnapshotInfo(C.Structure):
+    """rh_snapshot_info (rhydro_snapshot.h): SnapshotInfo plus the namelist
+    values the info file prints."""
+    _fields_ = [("t", C.c_double)] + [(k, C.c_int) for k in ("nstep", "nstep_coarse", "noutput", "iout", "nener")] + [
+        (k, C.c_double) for k in ("einit", "mass_tot_0", "rho_tot", "omega_m", "omega_l", "omega_k", "omega_b",
+                                  "h0", "aexp_ini", "boxlen_ini", "aexp", "hexp", "aexp_old", "epot_tot_int",
+                                  "epot_tot_old", "mass_sph")] + [
+        ("levelmin", C.c_int), ("levelmax", C.c_int)] + [
+        (k, C.c_double) for k in ("boxlen", "units_length", "units_density", "units_time")]
+
+
+class RhSnapshotTree(C.Structure):
+    """rh_snapshot_tree (rhydro_snapshot.h): the AmrGrid arrays the writers read."""
+    _fields_ = [(k, C.c_int) for k in ("nx", "ny", "nz", "ncoarse", "ngridmax", "ncell", "nlevelmax", "ncpu",
+                                       "nboundary")] + [
+        ("gamma", C.c_double), ("headl", C.POINTER(C.c_int)), ("taill", C.POINTER(C.c_int)),
+        ("numbl", C.POINTER(C.c_int)), ("headf", C.c_int), ("tailf", C.c_int), ("numbf", C.c_int)] + [
+        (k, C.POINTER(C.c_int)) for k in ("next", "prev", "father", "nbor", "son", "flag1", "cpu_map")] + [
+        ("xg", C.POINTER(C.c_double)), ("phi", C.POINTER(C.c_double)), ("f", C.POINTER(C.c_double)),
+        ("npart", C.c_longlong)]
+
+
+def snapshot_info(**kw) -> RhSnapshotInfo:
+    """SnapshotInfo's defaults (RamsesWriter.hpp:24-31: aexp 1, no outputs)
+    with unit scales 1 and boxlen 1, overridden by keyword."""
+    i = RhSnapshotInfo()
+    i.aexp_ini = i.boxlen_ini = i.aexp = i.aexp_old = 1.0
+    i.units_length = i.units_density = i.units_time = 1.0
+    i.boxlen = 1.0
+    for k, v in kw.items():
+        setattr(i, k, v)
+    return i
+
+
 def resolve_riemann(name: str | None) -> int:
     """Namelist string -> solver id, as the reference's face loop resolves it
     (HydroSolver.cpp:456-461): exact, case-sensitive matches, else LLF."""
@@ -124,6 +158,11 @@ def lib() -> C.CDLL:
         L.rh_bind_ensemble.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhMesh), C.c_int]
         L.rh_run_ensemble.argtypes = [C.c_void_p, _IP, _DP, C.c_double, C.c_int, _DP]
         L.rh_ensemble_leaf_cells.argtypes = [C.c_void_p, _IP, C.POINTER(C.c_longlong)]
+        L.rh_dump_snapshot.argtypes = [C.c_void_p, C.POINTER(RhSnapshotTree), C.POINTER(RhSnapshotInfo),
+                                       C.c_char_p]
+        L.rh_write_hydro.argtypes = [C.c_void_p, C.POINTER(RhSnapshotTree), C.POINTER(RhSnapshotInfo), C.c_char_p]
+        L.rh_write_amr.argtypes = [C.POINTER(RhSnapshotTree), C.POINTER(RhSnapshotInfo), C.c_char_p]
+        L.rh_amr_dump.argtypes = [C.c_void_p, C.POINTER(RhSnapshotInfo), C.c_int, C.c_char_p]
         _lib = L
     return _lib
 
@@ -135,7 +174,7 @@ EXPORTED = [
     "rh_level_leaf_cells", "rh_set_kernel_variant", "rh_get_kernel_variant", "rh_autotune", "rh_init_shock_tube",
     "rh_bind_ensemble", "rh_run_ensemble", "rh_ensemble_leaf_cells", "rh_add_gravity_source_terms",
     "rh_synchro_hydro_fine", "rh_amr_create", "rh_amr_destroy", "rh_amr_step", "rh_amr_mesh", "rh_amr_stats",
-    "rh_amr_context", "rh_amr_timers",
+    "rh_amr_context", "rh_amr_timers", "rh_dump_snapshot", "rh_write_hydro", "rh_write_amr", "rh_amr_dump",
 ]
 
 
@@ -214,6 +253,36 @@ class Mesh:
     def state(self, which: str = "uold") -> np.ndarray:
         return self.arrays[which].reshape(self.nvar, self.ncell)
 
+    def snapshot_tree(self, gamma: float, flag1=None, cpu_map=None, phi=None, f=None):
+        """(rh_snapshot_tree, arrays it points to) for the writers; flag1 and
+        cpu_map default to 0 and 1, phi and f to 0."""
+        a = self.arrays
+        t = RhSnapshotTree()
+        t.nx, t.ny, t.nz, t.ncoarse, t.ngridmax, t.ncell = self.nx, self.ny, self.nz, self.ncoarse, self.ngridmax, \
+            self.ncell
+        t.nlevelmax, t.ncpu, t.nboundary, t.gamma = self.nlevelmax, 1, 0, gamma
+        t.headf, t.tailf, t.numbf = (int(getattr(self, k, 0)) for k in ("headf", "tailf", "numbf"))
+        keep = []
+        for k in ("headl", "taill", "numbl", "next", "prev", "father", "nbor", "son"):
+            setattr(t, k, _ip(a[k]))
+        for k, v in (("flag1", flag1), ("cpu_map", cpu_map)):
+            if v is None:
+                setattr(t, k, C.cast(None, _IP))
+            else:
+                v = np.ascontiguousarray(v, np.int32)
+                keep.append(v)
+                setattr(t, k, _ip(v))
+        t.xg = _dp(a["xg"])
+        for k, v in (("phi", phi), ("f", f)):
+            if v is None:
+                setattr(t, k, C.cast(None, _DP))
+            else:
+                v = np.ascontiguousarray(v, np.float64)
+                keep.append(v)
+                setattr(t, k, _dp(v))
+        t.npart = 0
+        return t, keep
+
 
 def uniform_mesh(levelmin: int, nx: int = 1, ny: int = 1, nz: int = 1, ngridmax: int | None = None,
                  nlevelmax: int | None = None) -> Mesh:
@@ -229,8 +298,19 @@ def uniform_mesh(levelmin: int, nx: int = 1, ny: int = 1, nz: int = 1, ngridmax:
                             _ip(a["numbl"]), _dp(a["xg"]))
     if ng < 0:
         raise RuntimeError(f"rh_build_uniform failed ({ng})")
-    m.headf = int(ng) + 1 if ng < ngridmax else 0  # first grid of the free list
+    # The free list after the build: grids ng+1..ngridmax (AmrGrid.cpp:259-302).
+    m.headf = int(ng) + 1 if ng < ngridmax else 0
+    m.tailf = ngridmax if ng < ngridmax else 0
+    m.numbf = ngridmax - int(ng)
     return m
+
+
+def write_amr(mesh: Mesh, path: str, gamma: float = 1.4, flag1=None, cpu_map=None, **info) -> None:
+    """RamsesWriter::write_amr of a host mesh (no context needed)."""
+    t, keep = mesh.snapshot_tree(gamma, flag1, cpu_map)
+    st = lib().rh_write_amr(C.byref(t), C.byref(snapshot_info(**info)), os.fsencode(path))
+    if st != RH_OK:
+        raise RhError(st, lib().rh_last_error(None).decode())
 
 
 def init_regions(m: Mesh, regions: list[dict], gamma: float, boxlen: float = 1.0) -> None:
@@ -401,6 +481,18 @@ class Solver:
     def synchro_hydro_fine(self, ilevel, dt):
         self._chk(lib().rh_synchro_hydro_fine(self._ctx, ilevel, dt))
 
+    def dump_snapshot(self, mesh: Mesh, root: str, gamma: float | None = None, flag1=None, cpu_map=None, phi=None,
+                      f=None, **info):
+        """Simulation::dump_snapshot of the bound mesh into <root>/output_<iout>
+        (hydro records packed on the GPU from the device state)."""
+        t, keep = mesh.snapshot_tree(self.params.gamma if gamma is None else gamma, flag1, cpu_map, phi, f)
+        self._chk(lib().rh_dump_snapshot(self._ctx, C.byref(t), C.byref(snapshot_info(**info)), os.fsencode(root)))
+
+    def write_hydro(self, mesh: Mesh, path: str, gamma: float | None = None):
+        """RamsesWriter::write_hydro of the bound mesh alone."""
+        t, keep = mesh.snapshot_tree(self.params.gamma if gamma is None else gamma)
+        self._chk(lib().rh_write_hydro(self._ctx, C.byref(t), C.byref(snapshot_info()), os.fsencode(path)))
+
     def courant(self, ilevel, dx, gamma, courant_factor) -> float:
         out = C.c_double()
         self._chk(lib().rh_courant(self._ctx, ilevel, dx, gamma, courant_factor, C.byref(out)))
@@ -541,6 +633,11 @@ class AmrRun:
 
     def solver_context(self):
         return lib().rh_amr_context(self._h)
+
+    def dump(self, root: str, iout: int, **info):
+        """dump_snapshot of the run into <root>/output_<iout>; `info` sets the
+        SnapshotInfo fields the run does not supply (noutput, aexp, ...)."""
+        self._chk(lib().rh_amr_dump(self._h, C.byref(snapshot_info(**info)), iout, os.fsencode(root)))
 
     def close(self):
         if self._h:
