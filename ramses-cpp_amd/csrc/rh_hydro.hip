@@ -680,7 +680,7 @@ constexpr bool oct_ranked() {
 }
 
 template <int RIEMANN, bool MIXED, bool EXACT>
-__global__ __launch_bounds__(O_BLOCK) __attribute__((amdgpu_waves_per_eu(oct_ranked<RIEMANN, MIXED>() ? 3 : 1, 8)))
+__global__ __launch_bounds__(O_BLOCK) __attribute__((amdgpu_waves_per_eu(oct_ranked<RIEMANN, MIXED>() ? 3 : 2, 8)))
 void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                                                           double dx, double inv_dx) {
     using TC = typename std::conditional<MIXED, float, double>::type;
@@ -750,24 +750,91 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
             copy5(fallback, q);
         }
     };
+    auto to_prim = [&](const double u[5], TC q[5]) {
+        double qd[5];
+        ctoprim(u, qd, gd);
+        narrow(qd, q);
+    };
+
+    // Across the oct face along dimension d: the neighbour oct E (or a
+    // record), the cell X beyond the face and, when X's oct is a same-level
+    // leaf, the cell X2 beyond X.  The FP64 form gathers the next
+    // dimension's X and X2 while it computes the current one (its two waves
+    // per SIMD cannot hide the table -> grid -> uold chain of dependent
+    // loads on their own); the three-wave ranked form has no registers to
+    // spare and gathers them in place.
+    constexpr bool AHEAD = !RANKED;
+    struct Across {
+        int E;
+        int4 rc;
+        int Xcell, X2cell;
+    };
+    auto across = [&](int d) {
+        Across a;
+        const bool pd = (ic0 >> d) & 1;
+        const int icS = ic0 ^ (1 << d);
+        a.E = lv.nbo[(size_t)(2 * d + (pd ? 1 : 0)) * noct + p];
+        a.rc = make_int4(0, 0, 0, 0);
+        a.X2cell = 0;
+        if (a.E >= 0) {
+            const int g = lv.grid[a.E];
+            a.Xcell = cell_id(m, g, icS);
+            if (AHEAD && ((lv.leaf[a.E] >> icS) & 1)) a.X2cell = cell_id(m, g, ic0);
+        } else {
+            a.rc = lv.rec[-1 - a.E];
+            a.Xcell = a.rc.x;
+        }
+        return a;
+    };
+    auto gather = [&](const Across& a, double uX[5], double uX2[5]) {
+        if (a.Xcell > 0) load_u(m.uold, m.ncell, a.Xcell, uX);
+        if (a.X2cell > 0) load_u(m.uold, m.ncell, a.X2cell, uX2);
+    };
+    Across nx{};
+    double uXn[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, uX2n[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if constexpr (AHEAD) {
+        nx = across(0);
+        gather(nx, uXn, uX2n);
+    }
 
 #pragma unroll 1
     for (int d = 0; d < 3; ++d) {
         const int b = 1 << d;
         const bool par = (ic0 >> d) & 1;  // 0: sibling on the plus side, X on the minus side
         const int icS = ic0 ^ b;
-        const int E = lv.nbo[(size_t)(2 * d + (par ? 1 : 0)) * noct + p];
+        Across cur{};
+        double uX[5], uX2[5];
+        int E;
         int4 rc = make_int4(0, 0, 0, 0);
         int Xcell;
-        if (E >= 0) {
-            Xcell = cell_id(m, lv.grid[E], icS);
+        if constexpr (AHEAD) {
+            cur = nx;
+            copy5(uXn, uX);
+            copy5(uX2n, uX2);
+            if (d < 2) {
+                nx = across(d + 1);
+                gather(nx, uXn, uX2n);
+            }
+            E = cur.E;
+            rc = cur.rc;
+            Xcell = cur.Xcell;
         } else {
-            rc = lv.rec[-1 - E];
-            Xcell = rc.x;
+            E = lv.nbo[(size_t)(2 * d + (par ? 1 : 0)) * noct + p];
+            if (E >= 0) {
+                Xcell = cell_id(m, lv.grid[E], icS);
+            } else {
+                rc = lv.rec[-1 - E];
+                Xcell = rc.x;
+            }
         }
         TC qS[5], qX[5];
         shfl5(qC, qS, b);
-        prim(Xcell, qC, qX);
+        if constexpr (AHEAD) {
+            if (Xcell > 0) to_prim(uX, qX);
+            else copy5(qC, qX);
+        } else {
+            prim(Xcell, qC, qX);
+        }
         TC qL1[5], qR1[5];
         sel5(par, qS, qX, qL1);
         sel5(par, qX, qS, qR1);
@@ -798,9 +865,10 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         double Fin[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         long long Kin = -1;
         if (E >= 0) {
-            if ((lv.leaf[E] >> icS) & 1) {
+            if (AHEAD ? cur.X2cell > 0 : ((lv.leaf[E] >> icS) & 1) != 0) {
                 TC qX2[5];
-                prim(cell_id(m, lv.grid[E], ic0), qX, qX2);
+                if constexpr (AHEAD) to_prim(uX2, qX2);
+                else prim(cell_id(m, lv.grid[E], ic0), qX, qX2);
                 TC a5[5], c5[5];
                 sel5(par, qC, qX2, a5);
                 sel5(par, qX2, qC, c5);
