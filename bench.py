@@ -127,14 +127,16 @@ def measure_ensemble(args, rhydro, np):
     }
 
 
-def measure_amr(args, rhydro, np, which):
+def measure_amr(args, rhydro, np, which, precision=None):
     """Adaptive runs with the state on the GPU (rh_amr_*): the AMR Sedov blast
     (reference levels 7-10, err_grad_d = err_grad_p = 0.1) or the
     Kelvin-Helmholtz slab (levels 8-11), timed over whole coarse steps
     (hydro, refinement criteria, tree edits, prolongation, restriction), in
-    the FP64 form.  Cell-updates = leaf cells of every godunov_fine call.  The
-    first two coarse steps are also run through the oracle on one host core
-    and compared value for value."""
+    the FP64 form unless `precision` says otherwise.  Cell-updates = leaf
+    cells of every godunov_fine call.  For the Sedov run the first two coarse
+    steps are also run through the oracle on one host core and compared value
+    for value."""
+    precision = rhydro.RH_PREC_FP64 if precision is None else precision
     if which == "sedov":
         kw = dict(levelmin=7, levelmax=10, ngridmax=300000, err_grad_d=0.1, err_grad_p=0.1,
                   regions=rhydro.sedov_regions(7))
@@ -149,7 +151,7 @@ def measure_amr(args, rhydro, np, which):
     common = dict(gamma=1.4, slope_type=1, riemann="hllc", interpol_type=1, interpol_var=0)
     L = rhydro.lib()
     tc0 = time.perf_counter()
-    run = rhydro.AmrRun(**kw, **common)
+    run = rhydro.AmrRun(**kw, **common, precision=precision)
     t_init = time.perf_counter() - tc0
     ctx = run.solver_context()
     L.rh_timers_enable(ctx, 1)
@@ -165,13 +167,14 @@ def measure_amr(args, rhydro, np, which):
     L.rh_timers_read(ctx, rhydro.TK_GODUNOV, byref(ms), byref(n))
     st = run.state()
     run.close()
-    out = {"workload": name, "coarse_steps": steps, "fine_steps": nstep, "cell_updates": int(cu1 - cu0),
+    out = {"workload": name, "precision": "mixed" if precision == rhydro.RH_PREC_MIXED else "fp64",
+           "coarse_steps": steps, "fine_steps": nstep, "cell_updates": int(cu1 - cu0),
            "value": (cu1 - cu0) / wall, "wall_s": wall, "init_s": t_init, "godunov_ms_total": ms.value,
            "godunov_launches": n.value, "octs_per_level": st["numbl"].tolist(),
            "host_phase_ms": {k: ph1[k] - ph0[k] for k in ph1}}
     print(f"[bench] amr {which}: {steps} coarse steps, {cu1 - cu0} cell-updates in {wall:.3f}s "
           f"(godunov {ms.value:.1f} ms), lists {st['numbl'].tolist()}", file=sys.stderr, flush=True)
-    if not args.no_cpu and which == "sedov":
+    if not args.no_cpu and which == "sedov" and precision == rhydro.RH_PREC_FP64:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
 
@@ -256,7 +259,7 @@ def main() -> int:
     ap.add_argument("--no-sedov", action="store_true", help="skip the 64^3 Sedov run")
     ap.add_argument("--only", choices=["ensemble", "amr_sedov", "amr_kh", "sedov64"],
                     help="run one secondary workload alone and print its record (for profiling)")
-    ap.add_argument("--amr-steps", type=int, default=10, help="coarse steps of the AMR Sedov run (0: skip)")
+    ap.add_argument("--amr-steps", type=int, default=50, help="coarse steps of the AMR Sedov run (0: skip)")
     ap.add_argument("--kh-steps", type=int, default=1, help="coarse steps of the AMR Kelvin-Helmholtz run (0: skip)")
     args = ap.parse_args()
 
@@ -391,6 +394,10 @@ def main() -> int:
     sedov_info = measure_sedov64(args, rhydro, np) if not args.no_sedov else None
     amr_sedov = measure_amr(args, rhydro, np, "sedov") if args.amr_steps > 0 else None
     amr_kh = measure_amr(args, rhydro, np, "kh") if args.kh_steps > 0 else None
+    if amr_kh is not None:
+        m = measure_amr(args, rhydro, np, "kh", rhydro.RH_PREC_MIXED)
+        amr_kh["mixed"] = {k: m[k] for k in ("value", "wall_s", "cell_updates", "fine_steps", "godunov_ms_total",
+                                             "host_phase_ms")}
 
     value = nleaf * args.steps / fp64["wall"]
     achieved = fp64["achieved"]

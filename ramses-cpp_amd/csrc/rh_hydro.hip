@@ -338,55 +338,7 @@ RH_DEV void sort6(long long ks[6], int ix[6]) {
 #undef RH_CX
 }
 
-// Processing order of a cell's six incoming contributions, known from the
-// topology alone: keys[f] is the key of the contribution arriving through
-// face f (KMAX: none).  Returns the rank of each face's contribution packed
-// 3 bits per face, and the number of contributions ordered before the cell
-// itself (nbefore) and in total (nvalid).
 constexpr long long KMAX = 0x7fffffffffffffffLL;
-RH_DEV unsigned rank_contributions(const long long keys[6], long long own_key, int& nbefore, int& nvalid) {
-    long long ks[6];
-    int ix[6];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        ks[f] = keys[f];
-        ix[f] = f;
-    }
-    sort6(ks, ix);
-    const long long KOWN = own_key * 8 + 6;
-    unsigned pack = 0;
-    nbefore = 0;
-    nvalid = 0;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        pack |= (unsigned)k << (3 * ix[k]);
-        nbefore += ks[k] < KOWN ? 1 : 0;
-        nvalid += ks[k] != KMAX ? 1 : 0;
-    }
-    return pack;
-}
-
-// Ordered accumulation with the contributions already stored in processing
-// order (s_r rows 5k..5k+4 hold the k-th one): the first nbefore before the
-// cell's own update and clamps, the rest after (HydroSolver.cpp:476-547).
-template <int NT>
-RH_DEV void ordered_apply_ranked(const MeshDev& m, int cC, const double fs[5], double dtdx, const GasParams& gp,
-                                 double (*s_r)[NT], int t, int nbefore, int nvalid) {
-    double u[5];
-    load_u(m.unew, m.ncell, cC, u);
-    for (int k = 0; k < nbefore; ++k) {
-#pragma unroll
-        for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] - s_r[k * 5 + iv][t];
-    }
-    own_update(u, fs, dtdx, gp);
-    for (int k = nbefore; k < nvalid; ++k) {
-#pragma unroll
-        for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] - s_r[k * 5 + iv][t];
-    }
-    const size_t ci = (size_t)cC - 1;
-#pragma unroll
-    for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + ci] = u[iv];
-}
 
 // Ordered accumulation of one leaf cell (HydroSolver.cpp:476-547 in
 // processing order).  s_r/s_k hold, for this thread, the six incoming
@@ -395,7 +347,8 @@ RH_DEV void ordered_apply_ranked(const MeshDev& m, int cC, const double fs[5], d
 // clamps are applied, then the rest.
 template <int NT>
 RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const double fs[5], double dtdx,
-                          const GasParams& gp, double (*s_r)[NT], long long (*s_k)[NT], int t) {
+                          const GasParams& gp, double (*s_r)[NT], long long (*s_k)[NT], int t,
+                          const double* unew_pre = nullptr) {
     const long long KOWN = own_key * 8 + 6;
     long long ks[6];
     int ix[6];
@@ -413,7 +366,8 @@ RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const dou
         for (int iv = 0; iv < 5; ++iv) rs[k][iv] = s_r[ix[k] * 5 + iv][t];
     }
     double u[5];
-    load_u(m.unew, m.ncell, cC, u);
+    if (unew_pre) copy5(unew_pre, u);
+    else load_u(m.unew, m.ncell, cC, u);
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         const bool take = ks[k] < KOWN;
@@ -668,25 +622,17 @@ template <typename T> RH_DEV void sel5(bool c, const T a[5], const T b[5], T out
 constexpr int O_OCTS = 16;
 constexpr int O_BLOCK = O_OCTS * 8;
 
-// Occupancy: the mixed form fits three waves per SIMD in 168 VGPRs once its
-// LDS payload drops the keys: contributions are written in processing order
-// (RANKED) and 30 KB of LDS per workgroup lets five workgroups share a CU.
-// The mixed HLL solver would spill at that cap and stays at two waves; the
-// FP64 form is register-bound at two waves either way.  Those keep the keys
-// beside the contributions and sort them at the end (faster for them).
-template <int R, bool M>
-constexpr bool oct_ranked() {
-    return M && R != 1;
-}
-
+// Occupancy: both forms run two waves per SIMD (at most 256 VGPRs), with
+// 36 KB of LDS per workgroup for the incoming contributions and their keys.
+// The registers above what the arithmetic needs hold the next dimension's
+// neighbour states, gathered ahead of their use.
 template <int RIEMANN, bool MIXED, bool EXACT>
-__global__ __launch_bounds__(O_BLOCK) __attribute__((amdgpu_waves_per_eu(oct_ranked<RIEMANN, MIXED>() ? 3 : 2, 8)))
+__global__ __launch_bounds__(O_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                                                           double dx, double inv_dx) {
     using TC = typename std::conditional<MIXED, float, double>::type;
-    constexpr bool RANKED = oct_ranked<RIEMANN, MIXED>();
     __shared__ double s_r[30][O_BLOCK];
-    __shared__ long long s_k[RANKED ? 1 : 6][O_BLOCK];
+    __shared__ long long s_k[6][O_BLOCK];
     const int t = threadIdx.x;
     const int ic0 = t & 7;
     const int praw = blockIdx.x * O_OCTS + (t >> 3);
@@ -718,28 +664,6 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
     }
     double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
-    // Ranked form: the processing keys of the six incoming contributions
-    // follow from the topology alone, so their order is fixed before any flux
-    // is computed and each contribution goes straight into its rank's row.
-    int nbefore = 0, nvalid = 0;
-    unsigned rank_pack = 0;
-    if constexpr (RANKED) {
-        long long keys[6];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const bool par = (ic0 >> d) & 1;
-            const int icS = ic0 ^ (1 << d);
-            const int E = lv.nbo[(size_t)(2 * d + (par ? 1 : 0)) * noct + p];
-            const long long ksib = ((lm >> icS) & 1) ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : KMAX;
-            long long kx = KMAX;
-            if (E >= 0 && ((lv.leaf[E] >> icS) & 1)) kx = ((long long)E * 8 + icS) * 8 + (par ? 2 * d : 2 * d + 1);
-            // The sibling sits on the minus side when par = 1.
-            keys[2 * d] = par ? ksib : kx;
-            keys[2 * d + 1] = par ? kx : ksib;
-        }
-        rank_pack = rank_contributions(keys, (long long)p * 8 + ic0, nbefore, nvalid);
-    }
-
     auto prim = [&](int icell, const TC fallback[5], TC q[5]) {
         if (icell > 0) {
             double u[5], qd[5];
@@ -758,12 +682,9 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
 
     // Across the oct face along dimension d: the neighbour oct E (or a
     // record), the cell X beyond the face and, when X's oct is a same-level
-    // leaf, the cell X2 beyond X.  The FP64 form gathers the next
-    // dimension's X and X2 while it computes the current one (its two waves
-    // per SIMD cannot hide the table -> grid -> uold chain of dependent
-    // loads on their own); the three-wave ranked form has no registers to
-    // spare and gathers them in place.
-    constexpr bool AHEAD = !RANKED;
+    // leaf, the cell X2 beyond X.  The next dimension's X and X2 are
+    // gathered while the current one is computed: two waves per SIMD cannot
+    // hide the table -> grid -> uold chain of dependent loads on their own.
     struct Across {
         int E;
         int4 rc;
@@ -779,7 +700,7 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         if (a.E >= 0) {
             const int g = lv.grid[a.E];
             a.Xcell = cell_id(m, g, icS);
-            if (AHEAD && ((lv.leaf[a.E] >> icS) & 1)) a.X2cell = cell_id(m, g, ic0);
+            if ((lv.leaf[a.E] >> icS) & 1) a.X2cell = cell_id(m, g, ic0);
         } else {
             a.rc = lv.rec[-1 - a.E];
             a.Xcell = a.rc.x;
@@ -790,51 +711,37 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         if (a.Xcell > 0) load_u(m.uold, m.ncell, a.Xcell, uX);
         if (a.X2cell > 0) load_u(m.uold, m.ncell, a.X2cell, uX2);
     };
-    Across nx{};
+    Across nx = across(0);
     double uXn[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, uX2n[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if constexpr (AHEAD) {
-        nx = across(0);
-        gather(nx, uXn, uX2n);
-    }
+    gather(nx, uXn, uX2n);
 
-#pragma unroll 1
+    // The FP64 LLF and HLL forms run fastest with the dimension loop
+    // unrolled (rotations by a constant dimension are free); HLLC's code
+    // would triple and lose more in instruction fetch than it gains.
+    constexpr int DIM_UNROLL = (!MIXED && RIEMANN != 2) ? 3 : 1;
+#pragma unroll DIM_UNROLL
     for (int d = 0; d < 3; ++d) {
         const int b = 1 << d;
         const bool par = (ic0 >> d) & 1;  // 0: sibling on the plus side, X on the minus side
         const int icS = ic0 ^ b;
-        Across cur{};
+        const Across cur = nx;
         double uX[5], uX2[5];
-        int E;
-        int4 rc = make_int4(0, 0, 0, 0);
-        int Xcell;
-        if constexpr (AHEAD) {
-            cur = nx;
-            copy5(uXn, uX);
-            copy5(uX2n, uX2);
-            if (d < 2) {
-                nx = across(d + 1);
-                gather(nx, uXn, uX2n);
-            }
-            E = cur.E;
-            rc = cur.rc;
-            Xcell = cur.Xcell;
+        copy5(uXn, uX);
+        copy5(uX2n, uX2);
+        if (d < 2) {
+            nx = across(d + 1);
+            gather(nx, uXn, uX2n);
         } else {
-            E = lv.nbo[(size_t)(2 * d + (par ? 1 : 0)) * noct + p];
-            if (E >= 0) {
-                Xcell = cell_id(m, lv.grid[E], icS);
-            } else {
-                rc = lv.rec[-1 - E];
-                Xcell = rc.x;
-            }
+            // Last dimension: the cell's own unew entry (no other thread
+            // writes it in this kernel) takes the freed registers.
+            load_u(m.unew, m.ncell, cC, uXn);
         }
+        const int E = cur.E;
+        const int4 rc = cur.rc;
         TC qS[5], qX[5];
         shfl5(qC, qS, b);
-        if constexpr (AHEAD) {
-            if (Xcell > 0) to_prim(uX, qX);
-            else copy5(qC, qX);
-        } else {
-            prim(Xcell, qC, qX);
-        }
+        if (cur.Xcell > 0) to_prim(uX, qX);
+        else copy5(qC, qX);
         TC qL1[5], qR1[5];
         sel5(par, qS, qX, qL1);
         sel5(par, qX, qS, qR1);
@@ -865,10 +772,9 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         double Fin[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         long long Kin = -1;
         if (E >= 0) {
-            if (AHEAD ? cur.X2cell > 0 : ((lv.leaf[E] >> icS) & 1) != 0) {
+            if (cur.X2cell > 0) {
                 TC qX2[5];
-                if constexpr (AHEAD) to_prim(uX2, qX2);
-                else prim(cell_id(m, lv.grid[E], ic0), qX, qX2);
+                to_prim(uX2, qX2);
                 TC a5[5], c5[5];
                 sel5(par, qC, qX2, a5);
                 sel5(par, qX2, qC, c5);
@@ -940,21 +846,16 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         const int slotX = 2 * d + (par ? 1 : 0);
         const double sgS = par ? -1.0 : 1.0;
         const double sgX = par ? 1.0 : -1.0;
-        const int rowS = RANKED ? 5 * ((rank_pack >> (3 * slotS)) & 7) : 5 * slotS;
-        const int rowX = RANKED ? 5 * ((rank_pack >> (3 * slotX)) & 7) : 5 * slotX;
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
-            s_r[rowS + iv][t] = dtdx * sgS * Fsib[iv] * 0.125;
-            s_r[rowX + iv][t] = dtdx * sgX * Fin[iv] * 0.125;
+            s_r[5 * slotS + iv][t] = dtdx * sgS * Fsib[iv] * 0.125;
+            s_r[5 * slotX + iv][t] = dtdx * sgX * Fin[iv] * 0.125;
         }
-        if constexpr (!RANKED) {
-            s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
-            s_k[slotX][t] = Kin;
-        }
+        s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
+        s_k[slotX][t] = Kin;
     }
     if (!leaf) return;
-    if constexpr (RANKED) ordered_apply_ranked<O_BLOCK>(m, cC, fs, dtdx, gp, s_r, t, nbefore, nvalid);
-    else ordered_apply<O_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t);
+    ordered_apply<O_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn);
 }
 
 // Reflux into coarser leaf cells, one thread per target cell, contributions
