@@ -715,10 +715,10 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
     double uXn[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, uX2n[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     gather(nx, uXn, uX2n);
 
-    // The FP64 LLF and HLL forms run fastest with the dimension loop
-    // unrolled (rotations by a constant dimension are free); HLLC's code
+    // Unrolling the dimension loop makes the rotations by a constant
+    // dimension free; it pays for every form except FP64 HLLC, whose code
     // would triple and lose more in instruction fetch than it gains.
-    constexpr int DIM_UNROLL = (!MIXED && RIEMANN != 2) ? 3 : 1;
+    constexpr int DIM_UNROLL = (MIXED || RIEMANN != 2) ? 3 : 1;
 #pragma unroll DIM_UNROLL
     for (int d = 0; d < 3; ++d) {
         const int b = 1 << d;
