@@ -80,7 +80,8 @@ def measure_ensemble(args, rhydro, np):
         s.run_ensemble(levels, dxs, cf, args.steps)  # synchronises before returning
         wall = time.perf_counter() - t0
         k = {name: s.timer(tk) for name, tk in (("godunov", rhydro.TK_GODUNOV), ("courant", rhydro.TK_COURANT),
-                                                ("set_unew", rhydro.TK_SET_UNEW), ("set_uold", rhydro.TK_SET_UOLD))}
+                                                ("set_unew", rhydro.TK_SET_UNEW), ("set_uold", rhydro.TK_SET_UOLD),
+                                                ("step_join", rhydro.TK_STEP_JOIN))}
         s.timers(False)
         s.sync_to_host()
         s.close()
@@ -311,7 +312,8 @@ def main() -> int:
         dts = s.run_uniform(level, dx, cf, args.steps)  # synchronises before returning
         wall = time.perf_counter() - t0
         k = {name: s.timer(tk) for name, tk in (("godunov", rhydro.TK_GODUNOV), ("courant", rhydro.TK_COURANT),
-                                                ("set_unew", rhydro.TK_SET_UNEW), ("set_uold", rhydro.TK_SET_UOLD))}
+                                                ("set_unew", rhydro.TK_SET_UNEW), ("set_uold", rhydro.TK_SET_UOLD),
+                                                ("step_join", rhydro.TK_STEP_JOIN))}
         s.timers(False)
         s.sync_to_host()
         s.close()

@@ -66,7 +66,9 @@ int rh_init_lognormal(const rh_mesh* mesh, int level, unsigned long long seed, d
 
 /* Per-kernel device time accumulated on the context's stream (HIP events
  * around each launch) since the last reset: kernel 0 = godunov,
- * 1 = courant, 2 = set_unew, 3 = set_uold.  ms_total/count per kernel. */
+ * 1 = courant, 2 = set_unew, 3 = set_uold, 4 = the fused pass between two
+ * steps of rh_run_uniform (set_uold, courant and set_unew in one kernel).
+ * ms_total/count per kernel. */
 int rh_timers_enable(rh_ctx* ctx, int enable);
 int rh_timers_read(rh_ctx* ctx, int kernel, double* ms_total, long long* count);
 int rh_timers_reset(rh_ctx* ctx);

@@ -487,15 +487,24 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
     if ((st = get_level(c, ilevel, &lv))) return st;
     if ((st = ensure_dt(c, nsteps > 0 ? nsteps : 1))) return st;
     const rh::MeshDev m = mesh_dev(c);
+    // Step k is courant -> set_unew -> godunov_fine -> set_uold.  Between two
+    // steps, set_uold of the first and courant + set_unew of the second run
+    // as one fused pass (k_step_join), with the same state and time step.
     for (int k = 0; k < nsteps; ++k) {
-        if ((st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt + k))) return st;
-        {
+        if (k == 0) {
+            if ((st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt))) return st;
             Timed tm(c, TK_SET_UNEW);
             RH_HIP(c, rh::launch_set_unew(m, *lv, c->stream));
             tm.done();
         }
         if ((st = do_godunov(c, lv, c->d_dt + k, dx))) return st;
-        {
+        if (k + 1 < nsteps) {
+            if ((st = ensure_part(c, rh::courant_blocks(*lv) + 1))) return st;
+            Timed tm(c, TK_STEP_JOIN);
+            RH_HIP(c, rh::launch_step_join(m, *lv, dx, c->p.gamma, courant_factor, c->p.boxlen / 1e-10, c->d_part,
+                                           c->npart, c->d_dt + k + 1, c->stream));
+            tm.done();
+        } else {
             Timed tm(c, TK_SET_UOLD);
             RH_HIP(c, rh::launch_set_uold(m, *lv, c->stream));
             tm.done();

@@ -20,7 +20,7 @@ struct DevLevel {
     std::vector<void*> allocs;
 };
 
-enum { TK_GODUNOV = 0, TK_COURANT = 1, TK_SET_UNEW = 2, TK_SET_UOLD = 3, TK_N = 4 };
+enum { TK_GODUNOV = 0, TK_COURANT = 1, TK_SET_UNEW = 2, TK_SET_UOLD = 3, TK_STEP_JOIN = 4, TK_N = 5 };
 
 // One mesh of an ensemble: its host arrays and where its cells sit in the
 // packed device state (coarse cells from coff+1, grids from goff+1).
@@ -57,9 +57,9 @@ struct rh_ctx {
     int interpol_cap = 0;
     int variant = 1;  // Godunov kernel variant (rh_set_kernel_variant)
     bool timers = false;
-    double t_ms[rhi::TK_N] = {0, 0, 0, 0};
+    double t_ms[rhi::TK_N] = {0, 0, 0, 0, 0};
     double table_ms = 0.0;  // host time spent rebuilding level tables
-    long long t_n[rhi::TK_N] = {0, 0, 0, 0};
+    long long t_n[rhi::TK_N] = {0, 0, 0, 0, 0};
     // Event pairs recorded around launches, summed when the timers are read.
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;

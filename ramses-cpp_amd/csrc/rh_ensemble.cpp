@@ -282,16 +282,18 @@ int rh_run_ensemble(rh_ctx* c, const int* levels, const double* dx, double coura
     const rh::GasParams gp = gas(c, c->p.gamma);
     const double dt_init = c->p.boxlen / 1e-10;
     RH_HIP(c, rh::launch_fill_keys(c->d_keys, nk, dt_init, c->stream));
+    // As in rh_run_uniform, set_uold of one step and the Courant step and
+    // set_unew of the next run as one fused pass.
     for (int k = 0; k < nsteps; ++k) {
         unsigned long long* keys = c->d_keys + (size_t)k * nbox;
         double* dts = c->d_dt + (size_t)k * nbox;
-        {
-            Timed tm(c, TK_COURANT);
-            RH_HIP(c, rh::launch_courant_boxes(m, lv, c->p.gamma, courant_factor, keys, c->stream));
-            RH_HIP(c, rh::launch_keys_to_dt(keys, nbox, dts, c->stream));
-            tm.done();
-        }
-        {
+        if (k == 0) {
+            {
+                Timed tm(c, TK_COURANT);
+                RH_HIP(c, rh::launch_courant_boxes(m, lv, c->p.gamma, courant_factor, keys, c->stream));
+                RH_HIP(c, rh::launch_keys_to_dt(keys, nbox, dts, c->stream));
+                tm.done();
+            }
             Timed tm(c, TK_SET_UNEW);
             RH_HIP(c, rh::launch_set_unew(m, lv, c->stream));
             tm.done();
@@ -305,7 +307,13 @@ int rh_run_ensemble(rh_ctx* c, const int* levels, const double* dx, double coura
             if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
             tm.done();
         }
-        {
+        if (k + 1 < nsteps) {
+            unsigned long long* next_keys = keys + nbox;
+            Timed tm(c, TK_STEP_JOIN);
+            RH_HIP(c, rh::launch_courant_boxes(m, lv, c->p.gamma, courant_factor, next_keys, c->stream, true));
+            RH_HIP(c, rh::launch_keys_to_dt(next_keys, nbox, dts + nbox, c->stream));
+            tm.done();
+        } else {
             Timed tm(c, TK_SET_UOLD);
             RH_HIP(c, rh::launch_set_uold(m, lv, c->stream));
             tm.done();

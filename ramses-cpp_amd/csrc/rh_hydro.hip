@@ -140,9 +140,8 @@ RH_DEV double wave_min(double v) {
 }
 
 // Courant candidate of one leaf cell, HydroSolver.cpp:878-977.
-RH_DEV double courant_cand(const MeshDev& m, int id, double dx, double gamma, double courant_factor) {
-    double u[5];
-    load_u(m.uold, m.ncell, id, u);
+RH_DEV double courant_cand_u(const MeshDev& m, const double u[5], int id, double dx, double gamma,
+                             double courant_factor) {
     double d = smax(u[0], 1e-10), v2 = 0.0;
     for (int i = 1; i <= 3; ++i) {
         double v = u[i] / d;
@@ -164,6 +163,12 @@ RH_DEV double courant_cand(const MeshDev& m, int id, double dx, double gamma, do
     return (dx / u_wave) * (ssqrt(1.0 + 2.0 * courant_factor * alpha) - 1.0) / alpha;
 }
 
+RH_DEV double courant_cand(const MeshDev& m, int id, double dx, double gamma, double courant_factor) {
+    double u[5];
+    load_u(m.uold, m.ncell, id, u);
+    return courant_cand_u(m, u, id, dx, gamma, courant_factor);
+}
+
 // Per leaf cell, then a block minimum.
 __global__ __launch_bounds__(BLOCK) void k_courant_partial(MeshDev m, LevelDev lv, double dx, double gamma,
                                                            double courant_factor, double* part) {
@@ -174,6 +179,44 @@ __global__ __launch_bounds__(BLOCK) void k_courant_partial(MeshDev m, LevelDev l
     double cand = __builtin_inf();
     if (p < lv.noct && ((lv.leaf[p] >> ic0) & 1))
         cand = min_skip_nan(cand, courant_cand(m, cell_id(m, lv.grid[p], ic0), dx, gamma, courant_factor));
+    cand = wave_min(cand);
+    if ((t & 63) == 0) s_min[t >> 6] = cand;
+    __syncthreads();
+    if (t == 0) {
+        double v = s_min[0];
+        for (int w = 1; w < BLOCK / 64; ++w) v = min_skip_nan(v, s_min[w]);
+        part[blockIdx.x] = v;
+    }
+}
+
+// The end of one step of a level and the start of the next in one pass
+// over the level's cells: set_uold (uold <- unew on leaf cells), the Courant
+// candidates of the new state, then set_unew (unew <- uold; for a leaf cell
+// it already holds that value, so only non-leaf cells are written).  The
+// state and the block minima are exactly those of the three kernels run in
+// that order, for 80 B per leaf cell instead of 200.
+__global__ __launch_bounds__(BLOCK) void k_step_join(MeshDev m, LevelDev lv, double dx, double gamma,
+                                                     double courant_factor, double* part) {
+    __shared__ double s_min[BLOCK / 64];
+    const int t = threadIdx.x;
+    const int p = blockIdx.x * OCTS_PER_BLOCK + (t % OCTS_PER_BLOCK);
+    const int ic0 = t / OCTS_PER_BLOCK;
+    double cand = __builtin_inf();
+    if (p < lv.noct) {
+        const int id = cell_id(m, lv.grid[p], ic0);
+        const size_t c = (size_t)id - 1;
+        double u[5];
+        if ((lv.leaf[p] >> ic0) & 1) {
+            load_u(m.unew, m.ncell, id, u);
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) m.uold[(size_t)iv * m.ncell + c] = u[iv];
+            cand = min_skip_nan(cand, courant_cand_u(m, u, id, dx, gamma, courant_factor));
+        } else {
+            load_u(m.uold, m.ncell, id, u);
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
+        }
+    }
     cand = wave_min(cand);
     if ((t & 63) == 0) s_min[t >> 6] = cand;
     __syncthreads();
@@ -203,6 +246,10 @@ RH_DEV double key_dt(unsigned long long k) {
 // contiguous in the concatenated list, so a block whose first and last oct
 // share a box reduces in registers and LDS and issues one atomic; a block
 // straddling boxes lets every leaf cell issue its own.
+// JOIN: also the end of the previous step and the start of this one, as in
+// k_step_join (set_uold on leaf cells, the candidates from the new state,
+// set_unew on non-leaf cells).
+template <bool JOIN>
 __global__ __launch_bounds__(BLOCK) void k_courant_boxes(MeshDev m, LevelDev lv, double gamma, double courant_factor,
                                                          unsigned long long* keys) {
     __shared__ double s_min[BLOCK / 64];
@@ -214,7 +261,24 @@ __global__ __launch_bounds__(BLOCK) void k_courant_boxes(MeshDev m, LevelDev lv,
     const int b0 = lv.obox[p0], b1 = lv.obox[plast];
     double cand = __builtin_inf();
     int bx = b0;
-    if (p < lv.noct && ((lv.leaf[p] >> ic0) & 1)) {
+    if constexpr (JOIN) {
+        if (p < lv.noct) {
+            const int id = cell_id(m, lv.grid[p], ic0);
+            const size_t c = (size_t)id - 1;
+            double u[5];
+            if ((lv.leaf[p] >> ic0) & 1) {
+                bx = lv.obox[p];
+                load_u(m.unew, m.ncell, id, u);
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) m.uold[(size_t)iv * m.ncell + c] = u[iv];
+                cand = min_skip_nan(cand, courant_cand_u(m, u, id, lv.box_len[bx], gamma, courant_factor));
+            } else {
+                load_u(m.uold, m.ncell, id, u);
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
+            }
+        }
+    } else if (p < lv.noct && ((lv.leaf[p] >> ic0) & 1)) {
         bx = lv.obox[p];
         cand = min_skip_nan(cand, courant_cand(m, cell_id(m, lv.grid[p], ic0), lv.box_len[bx], gamma,
                                                courant_factor));
@@ -1189,6 +1253,16 @@ hipError_t launch_synchro(const MeshDev& m, const LevelDev& lv, double dt, doubl
     return hipGetLastError();
 }
 
+hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
+                            double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream) {
+    int nb = blocks_for(lv.noct);
+    if (nb > npart) return hipErrorInvalidValue;
+    if (nb > 0)
+        hipLaunchKernelGGL(k_step_join, dim3(nb), dim3(BLOCK), 0, stream, m, lv, dx, gamma, courant_factor, d_part);
+    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(256), 0, stream, (const double*)d_part, nb, dt_init, d_dt);
+    return hipGetLastError();
+}
+
 hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
                           double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream) {
     int nb = blocks_for(lv.noct);
@@ -1201,11 +1275,15 @@ hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, doubl
 }
 
 hipError_t launch_courant_boxes(const MeshDev& m, const LevelDev& lv, double gamma, double courant_factor,
-                                unsigned long long* keys, hipStream_t stream) {
+                                unsigned long long* keys, hipStream_t stream, bool join) {
     if (lv.noct == 0) return hipSuccess;
     if (!lv.obox || !lv.box_len) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_courant_boxes, dim3(blocks_for(lv.noct)), dim3(BLOCK), 0, stream, m, lv, gamma,
-                       courant_factor, keys);
+    if (join)
+        hipLaunchKernelGGL(k_courant_boxes<true>, dim3(blocks_for(lv.noct)), dim3(BLOCK), 0, stream, m, lv, gamma,
+                           courant_factor, keys);
+    else
+        hipLaunchKernelGGL(k_courant_boxes<false>, dim3(blocks_for(lv.noct)), dim3(BLOCK), 0, stream, m, lv, gamma,
+                           courant_factor, keys);
     return hipGetLastError();
 }
 

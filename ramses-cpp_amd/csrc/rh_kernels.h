@@ -61,6 +61,10 @@ hipError_t launch_synchro(const MeshDev& m, const LevelDev& lv, double dt, doubl
 hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
                           double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream);
 int courant_blocks(const LevelDev& lv);
+// set_uold of one step, then the Courant minimum (into *d_dt) and set_unew of
+// the next, fused into one pass over the level; same d_part requirement.
+hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
+                            double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream);
 // Godunov update of one level; dt is read from device memory (*d_dt).
 // variant 0: one thread per cell, octs strided across lanes; variant 1: the
 // eight cells of an oct in adjacent lanes, sharing by cross-lane shuffles.
@@ -72,8 +76,10 @@ hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams&
                           int variant, const double* d_dt, double dx, double inv_dx, hipStream_t stream);
 // Ensemble Courant step: keys[b] <- min(keys[b], key(dt candidate)) over the
 // leaf cells of box b (keys ordered like the doubles they encode, NaN last).
+// join: fused with set_uold before it and set_unew after it, as
+// launch_step_join.
 hipError_t launch_courant_boxes(const MeshDev& m, const LevelDev& lv, double gamma, double courant_factor,
-                                unsigned long long* keys, hipStream_t stream);
+                                unsigned long long* keys, hipStream_t stream, bool join = false);
 // keys[0..n) <- key(value).
 hipError_t launch_fill_keys(unsigned long long* keys, int n, double value, hipStream_t stream);
 // dt[i] <- the double encoded by keys[i], i < n.

@@ -21,7 +21,7 @@ RH_PREC_FP64, RH_PREC_MIXED = 0, 1
 RH_HOST_COHERENT, RH_DEVICE_RESIDENT = 0, 1
 RH_SYNC_TO_HOST, RH_SYNC_TO_DEVICE = 1, 2
 RH_UOLD, RH_UNEW = 1, 2
-TK_GODUNOV, TK_COURANT, TK_SET_UNEW, TK_SET_UOLD = range(4)
+TK_GODUNOV, TK_COURANT, TK_SET_UNEW, TK_SET_UOLD, TK_STEP_JOIN = range(5)
 
 _IP = C.POINTER(C.c_int)
 _DP = C.POINTER(C.c_double)
