@@ -22,10 +22,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -332,6 +336,65 @@ void nbor_exact(const rh_amr* a, int ig, int ic, int icn[6]) {
 
 // ---- host parallelism ------------------------------------------------------
 
+// Worker threads kept for the life of the process: a flag pass is a few
+// hundred microseconds of work, comparable to starting and joining threads,
+// so the passes share one pool instead of spawning their own.  run() hands
+// out chunks [k*chunk, (k+1)*chunk) through an atomic counter; the caller
+// works too and returns once every chunk is done.  One pass at a time (the
+// driver is single-threaded).
+class HostPool {
+  public:
+    explicit HostPool(int nworkers) {
+        for (int i = 0; i < nworkers; ++i) std::thread([this] { worker(); }).detach();
+    }
+    void run(int nchunks, const std::function<void(int)>& fn) {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            // A worker still leaving the previous pass may take a chunk of
+            // this one as soon as next_ is reset: everything it reads is set
+            // first (the chunk count never changes).
+            fn_.store(&fn);
+            nchunks_.store(nchunks);
+            left_.store(nchunks);
+            next_.store(0);
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [this] { return left_.load() == 0; });
+        fn_.store(nullptr);
+    }
+
+  private:
+    void work() {
+        for (int k; (k = next_.fetch_add(1)) < nchunks_.load();) {
+            (*fn_.load())(k);
+            if (left_.fetch_sub(1) == 1) {
+                std::lock_guard<std::mutex> lk(m_);
+                done_.notify_all();
+            }
+        }
+    }
+    void worker() {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (!fn_.load()) continue;
+            }
+            work();
+        }
+    }
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    std::atomic<const std::function<void(int)>*> fn_{nullptr};
+    std::atomic<int> nchunks_{0}, next_{0}, left_{0};
+    unsigned long long gen_ = 0;
+};
+
 // Runs fn(i) for i in [0, n) on the host's cores (the flag passes touch each
 // oct's own cells only, or defer their writes, so octs are independent).
 template <class Fn> void parallel_for(int n, Fn fn) {
@@ -343,16 +406,15 @@ template <class Fn> void parallel_for(int n, Fn fn) {
         for (int i = 0; i < n; ++i) fn(i);
         return;
     }
-    std::vector<std::thread> pool;
-    const int chunk = (n + nthreads - 1) / nthreads;
-    for (int k = 0; k < nthreads; ++k) {
+    // Never destroyed: its detached workers wait on it until the process ends.
+    static HostPool* pool = new HostPool(nthreads - 1);
+    const int nchunks = 4 * nthreads;
+    const int chunk = (n + nchunks - 1) / nchunks;
+    const std::function<void(int)> body = [&](int k) {
         const int lo = k * chunk, hi = std::min(n, lo + chunk);
-        if (lo >= hi) break;
-        pool.emplace_back([lo, hi, &fn] {
-            for (int i = lo; i < hi; ++i) fn(i);
-        });
-    }
-    for (auto& th : pool) th.join();
+        for (int i = lo; i < hi; ++i) fn(i);
+    };
+    pool->run(nchunks, body);
 }
 
 // The grids of a level list, in list order.

@@ -949,37 +949,43 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
     ordered_apply<O_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn);
 }
 
-// Reflux into coarser leaf cells, one thread per target cell, contributions
-// in processing order.  The slot indices and values of up to eight
-// contributions are loaded together before they are subtracted in order, so
-// a target with many contributions waits for a few memory round trips, not
-// one per contribution.
-__global__ __launch_bounds__(256) void k_coarse_reflux(MeshDev m, LevelDev lv) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
+// Reflux into coarser leaf cells, one wave per target cell, contributions in
+// processing order.  A coarse cell next to a finer region can receive dozens
+// of contributions; the wave loads up to 64 of them at once (slot index,
+// then the five values) into LDS and lane 0 subtracts them in order, so a
+// target costs two memory round trips per 64 contributions instead of a
+// chain of dependent loads.
+__global__ __launch_bounds__(64) void k_coarse_reflux(MeshDev m, LevelDev lv) {
+    __shared__ double s_v[64][5];
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x;
     if (t >= lv.ntgt) return;
     const size_t c = (size_t)lv.tgt_cell[t] - 1;
     double u[5];
+    if (lane == 0) {
 #pragma unroll
-    for (int iv = 0; iv < 5; ++iv) u[iv] = m.unew[(size_t)iv * m.ncell + c];
-    const int end = lv.tgt_off[t + 1];
-    for (int k = lv.tgt_off[t]; k < end; k += 8) {
-        const int n = min(8, end - k);
-        int idx[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) idx[j] = j < n ? lv.tgt_slot[k + j] : 0;
-        double v[8][5];
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-#pragma unroll
-            for (int iv = 0; iv < 5; ++iv) v[j][iv] = j < n ? lv.slot_r[(size_t)idx[j] * 5 + iv] : 0.0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (j < n)
-#pragma unroll
-                for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] - v[j][iv];
+        for (int iv = 0; iv < 5; ++iv) u[iv] = m.unew[(size_t)iv * m.ncell + c];
     }
+    const int beg = lv.tgt_off[t], end = lv.tgt_off[t + 1];
+    for (int k = beg; k < end; k += 64) {
+        const int n = min(64, end - k);
+        if (lane < n) {
+            const size_t slot = (size_t)lv.tgt_slot[k + lane];
 #pragma unroll
-    for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
+            for (int iv = 0; iv < 5; ++iv) s_v[lane][iv] = lv.slot_r[slot * 5 + iv];
+        }
+        __syncthreads();
+        if (lane == 0) {
+            for (int j = 0; j < n; ++j)
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) u[iv] = u[iv] - s_v[j][iv];
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
+    }
 }
 
 // HydroSolver.cpp:734-876 for nvar = 5: prolongation of one coarse cell and
@@ -1367,7 +1373,7 @@ hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams&
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || lv.ntgt == 0) return e;
-    hipLaunchKernelGGL(k_coarse_reflux, dim3((lv.ntgt + 255) / 256), dim3(256), 0, stream, m, lv);
+    hipLaunchKernelGGL(k_coarse_reflux, dim3(lv.ntgt), dim3(64), 0, stream, m, lv);
     return hipGetLastError();
 }
 
