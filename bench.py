@@ -50,6 +50,21 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def state_errors(A, B, gamma, np):
+    """L1 (mean) and Linf relative errors of rho, p and |v| of conservative
+    states A against B ([5, ncells] each); |v| relative to B's largest."""
+    out = {}
+    va, vb = A[1:4] / A[0], B[1:4] / B[0]
+    pa = (gamma - 1) * (A[4] - 0.5 * A[0] * (va ** 2).sum(0))
+    pb = (gamma - 1) * (B[4] - 0.5 * B[0] * (vb ** 2).sum(0))
+    sa, sb = np.sqrt((va ** 2).sum(0)), np.sqrt((vb ** 2).sum(0))
+    for name, x, y, scale in (("rho", A[0], B[0], np.abs(B[0])), ("p", pa, pb, np.abs(pb)),
+                              ("v", sa, sb, np.full_like(sb, max(sb.max(), 1e-300)))):
+        r = np.abs(x - y) / scale
+        out[name] = {"L1": float(r.mean()), "Linf": float(r.max())}
+    return out
+
+
 def measure_ensemble(args, rhydro, np):
     """The batched form on the shock-tube ensemble: `boxes` independent
     periodic boxes of 16-64 cells a side, each with its own level, cell width
@@ -112,6 +127,20 @@ def measure_ensemble(args, rhydro, np):
     s.close()
     unbatched = nleaf * args.steps / t_loop
     print(f"[bench] unbatched mixed loop: {t_loop:.4f}s, identical={same}", file=sys.stderr, flush=True)
+    # Per-box error of the mixed form against the FP64 form (bit-identical to
+    # the oracle) after warmup + steps, each form with its own time steps.
+    per_box = []
+    for b, m in enumerate(meshes):
+        cells = m.cells_of_level(levels[b], leaf_only=True)
+        A = finals["mixed"][b].reshape(5, -1)[:, cells]
+        B = finals["fp64"][b].reshape(5, -1)[:, cells]
+        per_box.append(state_errors(A, B, gamma, np))
+    err_summary = {}
+    for name in ("rho", "p", "v"):
+        for norm in ("L1", "Linf"):
+            vals = np.array([e[name][norm] for e in per_box])
+            err_summary[f"{name}_{norm}"] = {"max": float(vals.max()), "median": float(np.median(vals)),
+                                             "worst_box": int(vals.argmax())}
     return {
         "workload": f"shock-tube ensemble: {args.boxes} periodic cubic boxes, sides {sides[0]}-{sides[-1]} cells "
                     f"(log-uniform, nx*2^(L-1) with nx in 1,3,5,7; seed {args.seed}), {nleaf} leaf cells, "
@@ -125,6 +154,9 @@ def measure_ensemble(args, rhydro, np):
         "unbatched_mixed_value": unbatched,
         "batched_speedup": out["mixed"]["value"] / unbatched,
         "batched_equals_unbatched": same,
+        "mixed_error_vs_fp64_per_box": err_summary,
+        "error_note": "relative errors of the mixed form against the FP64 form on each box's leaf cells after "
+                      "warmup + steps, each form with its own time steps; max and median over the boxes",
     }
 
 
@@ -172,7 +204,7 @@ def measure_amr(args, rhydro, np, which, precision=None):
            "coarse_steps": steps, "fine_steps": nstep, "cell_updates": int(cu1 - cu0),
            "value": (cu1 - cu0) / wall, "wall_s": wall, "init_s": t_init, "godunov_ms_total": ms.value,
            "godunov_launches": n.value, "octs_per_level": st["numbl"].tolist(),
-           "host_phase_ms": {k: ph1[k] - ph0[k] for k in ph1}}
+           "host_phase_ms": {k: ph1[k] - ph0[k] for k in ph1}, "_state": st}
     print(f"[bench] amr {which}: {steps} coarse steps, {cu1 - cu0} cell-updates in {wall:.3f}s "
           f"(godunov {ms.value:.1f} ms), lists {st['numbl'].tolist()}", file=sys.stderr, flush=True)
     if not args.no_cpu and which == "sedov" and precision == rhydro.RH_PREC_FP64:
@@ -195,6 +227,34 @@ def measure_amr(args, rhydro, np, which, precision=None):
         g.close()
         o.close()
         print(f"[bench] amr {which}: oracle {ncmp} steps {tcpu:.1f}s identical={same}", file=sys.stderr, flush=True)
+    return out
+
+
+def amr_errors(ref, other, gamma, np):
+    """Mixed against FP64 after an adaptive run: the refinement decisions of
+    the two forms can differ, so the errors are taken on the leaf cells only
+    when both trees are identical."""
+    same_tree = all(np.array_equal(ref[k], other[k]) for k in ("numbl", "headl", "next", "son", "father"))
+    out = {"mesh_identical_to_fp64": bool(same_tree)}
+    if not same_tree:
+        out["cells_refined_differently"] = int((ref["son"] != other["son"]).sum())
+        return out
+    leaf = np.flatnonzero(ref["son"][ref["ncoarse"]:] == 0) + ref["ncoarse"]
+    inlist = np.zeros(ref["ncell"], bool)
+    for L in range(1, ref["nlevelmax"] + 1):
+        g, seq = int(ref["headl"][L]), []
+        while g > 0:
+            seq.append(g)
+            g = int(ref["next"][g - 1])
+        if seq:
+            gi = np.array(seq)
+            for ic in range(8):
+                inlist[ref["ncoarse"] + ic * ref["ngridmax"] + gi - 1] = True
+    cells = leaf[inlist[leaf]]
+    A = other["uold"].reshape(5, -1)[:, cells]
+    B = ref["uold"].reshape(5, -1)[:, cells]
+    out["error_vs_fp64"] = state_errors(A, B, gamma, np)
+    out["error_note"] = "relative errors of the mixed form against the FP64 form on all leaf cells after the run"
     return out
 
 
@@ -280,6 +340,7 @@ def main() -> int:
             rec = measure_sedov64(args, rhydro, np)
         else:
             rec = measure_amr(args, rhydro, np, "sedov" if args.only == "amr_sedov" else "kh")
+            rec.pop("_state", None)
         print(json.dumps({"only": args.only, "record": rec}))
         return 0
 
@@ -343,17 +404,7 @@ def main() -> int:
     def errors(a, b):
         """L1 (mean) and Linf relative errors of rho, p, |v| of state a against b."""
         cells = mesh.cells_of_level(level)
-        A, B = a.reshape(5, -1)[:, cells], b.reshape(5, -1)[:, cells]
-        out = {}
-        va, vb = A[1:4] / A[0], B[1:4] / B[0]
-        pa = (gamma - 1) * (A[4] - 0.5 * A[0] * (va ** 2).sum(0))
-        pb = (gamma - 1) * (B[4] - 0.5 * B[0] * (vb ** 2).sum(0))
-        sa, sb = np.sqrt((va ** 2).sum(0)), np.sqrt((vb ** 2).sum(0))
-        for name, x, y, scale in (("rho", A[0], B[0], np.abs(B[0])), ("p", pa, pb, np.abs(pb)),
-                                  ("v", sa, sb, np.full_like(sb, sb.max()))):
-            r = np.abs(x - y) / scale
-            out[name] = {"L1": float(r.mean()), "Linf": float(r.max())}
-        return out
+        return state_errors(a.reshape(5, -1)[:, cells], b.reshape(5, -1)[:, cells], gamma, np)
 
     mixed_info = None
     if mixed is not None:
@@ -400,6 +451,10 @@ def main() -> int:
         m = measure_amr(args, rhydro, np, "kh", rhydro.RH_PREC_MIXED)
         amr_kh["mixed"] = {k: m[k] for k in ("value", "wall_s", "cell_updates", "fine_steps", "godunov_ms_total",
                                              "host_phase_ms")}
+        amr_kh["mixed"].update(amr_errors(amr_kh["_state"], m["_state"], 1.4, np))
+    for rec in (amr_sedov, amr_kh):
+        if rec is not None:
+            rec.pop("_state", None)
 
     value = nleaf * args.steps / fp64["wall"]
     achieved = fp64["achieved"]
