@@ -198,3 +198,61 @@ def test_autotune_keeps_state_and_results():
     s.sync_to_host()
     assert_same(m.uold, mo.uold, "uold after tuned steps")
     s.close()
+
+
+def test_empty_levels_are_no_ops():
+    """Level lists without octs (above the built levels): the Courant step is
+    the initial value, set_unew/godunov_fine/set_uold change nothing, as in
+    the oracle."""
+    m = rhydro.uniform_mesh(3, nlevelmax=5)
+    rhydro.init_lognormal(m, 3, 3, 1.4)
+    mo = clone(m)
+    before = m.uold.copy()
+    s = rhydro.Solver(gamma=1.4, slope_type=1, riemann="hllc")
+    s.bind(m)
+    om = oracle.OracleMesh(mo)
+    p = oracle_params(s)
+    print("case empty levels 4 and 5", flush=True)
+    for level in (4, 5):
+        dtg = step_gpu(s, level, 1.0 / 2 ** level, 0.8)
+        dto = step_oracle(om, p, level, 1.0 / 2 ** level, 0.8)
+        assert dtg == dto, (level, dtg, dto)
+    s.sync_to_host()
+    assert_same(m.uold, before, "uold unchanged")
+    assert_same(m.uold, mo.uold, "uold")
+    assert_same(m.unew, mo.unew, "unew")
+    s.close()
+
+
+@pytest.mark.parametrize("riemann", ["llf", "hllc"])
+def test_nonfinite_and_degenerate_cells(riemann):
+    """NaN and infinite densities and energies, zero and negative densities,
+    negative pressures: the GPU path propagates them exactly as the oracle
+    (NaN where the oracle has NaN, the same floors and clamps elsewhere), and
+    the Courant step skips NaN candidates as std::min does."""
+    level = 4
+    m = _random_mesh(level, seed=99, gamma=1.4)
+    cells = m.cells_of_level(level)
+    u = m.state("uold")
+    rng = np.random.default_rng(7)
+    pick = rng.choice(cells.size, 24, replace=False)
+    c = cells[pick]
+    u[0, c[0:3]] = np.nan
+    u[4, c[3:6]] = np.inf
+    u[0, c[6:9]] = 0.0
+    u[0, c[9:12]] = -1.0
+    u[4, c[12:15]] = -5.0
+    u[1, c[15:18]] = -np.inf
+    u[0, c[18:21]] = 1e-300
+    u[4, c[21:24]] = np.nan
+    m.unew[:] = m.uold
+    _compare_steps(m, dict(gamma=1.4, slope_type=1, riemann=riemann), level, 2)
+
+
+def test_amr_run_reports_exhausted_grids():
+    """A run whose refinement needs more grids than ngridmax stops with an
+    error (the driver does not resize), not a fault."""
+    print("case amr run ngridmax too small", flush=True)
+    with pytest.raises(rhydro.RhError):
+        rhydro.AmrRun(levelmin=4, levelmax=6, ngridmax=80, err_grad_d=0.1, err_grad_p=0.1,
+                      regions=rhydro.sedov_regions(4))
