@@ -290,8 +290,65 @@ int find_cell(const rh_amr* a, const double x[3], int level) {
     return cur;
 }
 
+// The cell at offset o (each -1, 0, 1) from cell ic0 of oct g, at the same
+// level, reached through the nbor/son links: 0 when a link does not lead to
+// an oct at exactly the expected position (a coarser region, or a diagonal
+// reached only around a coarser one); the caller then asks find_cell.  The
+// expected positions are exact (xg values are dyadic).
+int walk_neighbour(const rh_amr* a, int g, int ic0, const int o[3], double w) {
+    const int n[3] = {a->nx, a->ny, a->nz};
+    int bits = 0;
+    for (int d = 0; d < 3; ++d) {
+        const int q = ((ic0 >> d) & 1) + o[d];
+        const int s = q < 0 ? -1 : (q > 1 ? 1 : 0);
+        bits |= (q & 1) << d;
+        if (s == 0) continue;
+        const int c1 = a->nbor[(size_t)(2 * d + (s > 0 ? 1 : 0)) * a->ngridmax + g - 1];
+        if (c1 <= 0 || c1 > a->ncell) return 0;
+        const int g2 = a->son[c1 - 1];
+        if (g2 <= 0) return 0;
+        for (int e = 0; e < 3; ++e) {
+            double want = a->xg[(size_t)e * a->ngridmax + g - 1] + (e == d ? s * w : 0.0);
+            if (want < 0.0) want += n[e];
+            else if (want >= n[e]) want -= n[e];
+            if (a->xg[(size_t)e * a->ngridmax + g2 - 1] != want) return 0;
+        }
+        g = g2;
+    }
+    return cell(a, g, bits + 1);
+}
+
 // Every 27-neighbour of a cell exists at its level and is refined.
+bool neighbours_refined_by_search(const rh_amr* a, int icell);
+
 bool neighbours_refined(const rh_amr* a, int icell) {
+    // The coordinate search scales positions by boxlen/max(nx,ny,nz) at the
+    // coarse level and by boxlen/nx below it; the two agree with the links
+    // only when the coarse grid is cubic.
+    if (icell <= a->ncoarse || a->nx != a->ny || a->nx != a->nz) return neighbours_refined_by_search(a, icell);
+    const int g = ((icell - a->ncoarse - 1) % a->ngridmax) + 1;
+    const int ic0 = (icell - a->ncoarse - 1) / a->ngridmax;
+    // Width of an oct of this cell's depth D (father chain) in coarse-cell
+    // units: a depth-1 oct fills one coarse cell, and each refinement halves
+    // it (make_grid_fine's offsets), so 1/2^(D-1).
+    const int level = depth_of(a, g);
+    const double w = 2.0 / (double)(1 << level);
+    bool any_fallback = false;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int o[3] = {dx, dy, dz};
+                const int c = walk_neighbour(a, g, ic0, o, w);
+                if (c == 0) {
+                    any_fallback = true;
+                    continue;
+                }
+                if (a->son[c - 1] == 0) return false;
+            }
+    return any_fallback ? neighbours_refined_by_search(a, icell) : true;
+}
+
+bool neighbours_refined_by_search(const rh_amr* a, int icell) {
     int level = cell_level(a, icell);
     double xc[3];
     cell_center(a, icell, xc);
