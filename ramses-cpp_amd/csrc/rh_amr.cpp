@@ -136,15 +136,19 @@ void refresh_mesh(rh_amr* a) {
     m.mesh_epoch = a->epoch;
 }
 
-// The tree changed: the context's level tables and the AMR kernels' copies
-// of father/nbor are stale.
+// The tree changed: the context's level tables are stale.  The device
+// copies of father/nbor stay current without an upload: kernels read them
+// only for listed grids, k_new_octs writes the entries of every grid it
+// creates, and a removed grid's stale entries are unused until a new oct
+// takes the grid (and writes them again).
 void tree_changed(rh_amr* a) {
     a->epoch++;
     a->ctx->mesh.mesh_epoch = a->epoch;
     a->mesh.mesh_epoch = a->epoch;
-    a->topo_dirty = true;
 }
 
+// The whole host father/nbor once, before the first kernel that reads them
+// (the octs made before it are in the host arrays too).
 int upload_topology(rh_amr* a) {
     if (!a->topo_dirty) return RH_OK;
     RH_AHIP(a, hipMemcpyAsync(a->d_father, a->father.data(), (size_t)a->ngridmax * sizeof(int),
@@ -368,29 +372,6 @@ bool neighbours_refined_by_search(const rh_amr* a, int icell) {
     return true;
 }
 
-// get_nbor_cells_exact (AmrGrid.cpp:449-472): same-level neighbour cells of
-// cell ic (1-based) of grid ig, 0 where the neighbour oct does not exist.
-void nbor_exact(const rh_amr* a, int ig, int ic, int icn[6]) {
-    const int ic0 = ic - 1;
-    for (int d = 0; d < 3; ++d)
-        for (int side = 0; side < 2; ++side) {
-            const int par = (ic0 >> d) & 1;
-            const int icS = (ic0 ^ (1 << d)) + 1;
-            int v = 0;
-            if (par != side) {
-                v = cell(a, ig, icS);
-            } else {
-                int nb = a->nbor[(size_t)(2 * d + side) * a->ngridmax + ig - 1];
-                if (nb > a->ncell) nb = 0;
-                if (nb > 0) {
-                    int child = a->son[nb - 1];
-                    if (child > 0) v = cell(a, child, icS);
-                }
-            }
-            icn[2 * d + side] = v;
-        }
-}
-
 // ---- host parallelism ------------------------------------------------------
 
 // Worker threads kept for the life of the process: a flag pass is a few
@@ -494,7 +475,17 @@ bool child_active(const rh_amr* a, int idc0) {
     return false;
 }
 
-void smooth_fine(rh_amr* a, int il) {
+// The six same-level neighbour octs of oct ig (0 where none), face order
+// 2*dim + side: the son of each father-level neighbour cell
+// (get_nbor_cells_exact, AmrGrid.cpp:449-472, takes its cells from them).
+void nbor_octs(const rh_amr* a, int ig, int out[6]) {
+    for (int f = 0; f < 6; ++f) {
+        int nb = a->nbor[(size_t)f * a->ngridmax + ig - 1];
+        out[f] = (nb > 0 && nb <= a->ncell) ? a->son[nb - 1] : 0;
+    }
+}
+
+void smooth_fine(rh_amr* a, int il, const std::vector<int>& grids) {
     if (il == 1) {
         std::vector<int> ok(a->ncoarse, 0);
         for (int ic = 1; ic <= a->ncoarse; ++ic) {
@@ -510,29 +501,43 @@ void smooth_fine(rh_amr* a, int il) {
             if (ok[i]) a->flag1[i] = 1;
         return;
     }
-    const std::vector<int> grids = list_grids(a, il);
+    // Two passes over the octs: which unflagged cells have a flagged
+    // same-level face neighbour (from the flags before this pass), then
+    // flag them.  A cell's neighbour along d is its sibling inside the oct
+    // or the cell of the neighbour oct on that side (get_nbor_cells_exact),
+    // each oct's six neighbour octs looked up once.
     std::vector<unsigned char> ok(grids.size(), 0);  // bit ic-1 of each oct
     parallel_for((int)grids.size(), [&](int k) {
         const int ig = grids[k];
+        int on[6];
+        bool looked = false;
         unsigned char bits = 0;
         for (int ic = 1; ic <= 8; ++ic) {
-            int idc = cell(a, ig, ic) - 1;
-            if (a->flag1[idc] != 0) continue;
-            int icn[6];
-            nbor_exact(a, ig, ic, icn);
-            int nf = 0;
-            for (int f = 0; f < 6; ++f)
-                if (icn[f] > 0 && icn[f] <= a->ncell && a->flag1[icn[f] - 1] == 1) nf++;
-            if (nf > 0) bits |= (unsigned char)(1u << (ic - 1));
+            if (a->flag1[cell(a, ig, ic) - 1] != 0) continue;
+            if (!looked) {
+                nbor_octs(a, ig, on);
+                looked = true;
+            }
+            const int ic0 = ic - 1;
+            bool any = false;
+            for (int d = 0; d < 3 && !any; ++d)
+                for (int side = 0; side < 2 && !any; ++side) {
+                    const int icS = (ic0 ^ (1 << d)) + 1;
+                    const int par = (ic0 >> d) & 1;
+                    const int og = par != side ? ig : on[2 * d + side];
+                    any = og > 0 && a->flag1[cell(a, og, icS) - 1] == 1;
+                }
+            if (any) bits |= (unsigned char)(1u << ic0);
         }
         ok[k] = bits;
     });
-    for (size_t k = 0; k < grids.size(); ++k)
+    parallel_for((int)grids.size(), [&](int k) {
         for (int ic = 1; ic <= 8; ++ic)
             if ((ok[k] >> (ic - 1)) & 1) a->flag1[cell(a, grids[k], ic) - 1] = 1;
+    });
 }
 
-void test_flag(rh_amr* a, int il) {
+void test_flag(rh_amr* a, int il, const std::vector<int>& grids) {
     if (il == 1) {
         for (int ic = 1; ic <= a->ncoarse; ++ic)
             if (child_active(a, ic - 1)) a->flag1[ic - 1] = 1;
@@ -540,7 +545,6 @@ void test_flag(rh_amr* a, int il) {
     }
     // Each cell's test reads only its own children, which are not flagged
     // here, so the octs are independent.
-    const std::vector<int> grids = list_grids(a, il);
     parallel_for((int)grids.size(), [&](int k) {
         for (int ic = 1; ic <= 8; ++ic) {
             int idc = cell(a, grids[k], ic) - 1;
@@ -549,8 +553,7 @@ void test_flag(rh_amr* a, int il) {
     });
 }
 
-void ensure_ref_rules(rh_amr* a, int il) {
-    const std::vector<int> grids = list_grids(a, il);
+void ensure_ref_rules(rh_amr* a, const std::vector<int>& grids) {
     parallel_for((int)grids.size(), [&](int k) {
         const int ig = grids[k];
         int fc = a->father[ig - 1];
@@ -625,7 +628,7 @@ int coarse_gradients(rh_amr* a) {
 }
 
 // Gradient criteria of a level list on the device; flags come back per cell.
-int list_gradients(rh_amr* a, int il) {
+int list_gradients(rh_amr* a, int il, const std::vector<int>& grids) {
     const rh::LevelDev* lv;
     int st = get_level(a->ctx, il, &lv);
     if (st) return st;
@@ -636,7 +639,6 @@ int list_gradients(rh_amr* a, int il) {
     std::vector<unsigned char> fl((size_t)lv->noct * 8);
     RH_AHIP(a, hipMemcpyAsync(fl.data(), a->d_flags, fl.size(), hipMemcpyDeviceToHost, a->ctx->stream));
     RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
-    const std::vector<int> grids = list_grids(a, il);
     parallel_for((int)grids.size(), [&](int pos) {
         for (int ic = 1; ic <= 8; ++ic) {
             int idc = cell(a, grids[pos], ic) - 1;
@@ -650,12 +652,13 @@ int list_gradients(rh_amr* a, int il) {
 int flag_fine(rh_amr* a, int il, int icount, int nsub) {
     if (il >= a->nlevelmax) return RH_OK;
     const int lmin = a->levelmin;
+    // The list does not change during the passes (they only set flags).
+    const std::vector<int> grids = il == 1 ? std::vector<int>() : list_grids(a, il);
     if (il == 1) {
         for (int i = 0; i < a->ncoarse; ++i) a->flag1[i] = 0;
         for (int i = 1; i <= a->ncoarse; ++i)
             if (il < lmin || child_active(a, i - 1)) a->flag1[i - 1] = 1;
     } else {
-        const std::vector<int> grids = list_grids(a, il);
         parallel_for((int)grids.size(), [&](int k) {
             for (int ic = 1; ic <= 8; ++ic) {
                 int idc = cell(a, grids[k], ic) - 1;
@@ -663,12 +666,12 @@ int flag_fine(rh_amr* a, int il, int icount, int nsub) {
             }
         });
     }
-    if (il >= lmin) smooth_fine(a, il);
+    if (il >= lmin) smooth_fine(a, il, grids);
     if (il >= lmin && (a->ap.err_grad_d > 0.0 || a->ap.err_grad_p > 0.0 || a->ap.err_grad_v > 0.0))
-        RH_ACHK(il == 1 ? coarse_gradients(a) : list_gradients(a, il));
-    if (il > lmin && icount < nsub) test_flag(a, il);
-    for (int i = 0; i < a->ap.nexpand; ++i) smooth_fine(a, il);
-    if (il > lmin && icount < nsub) ensure_ref_rules(a, il);
+        RH_ACHK(il == 1 ? coarse_gradients(a) : list_gradients(a, il, grids));
+    if (il > lmin && icount < nsub) test_flag(a, il, grids);
+    for (int i = 0; i < a->ap.nexpand; ++i) smooth_fine(a, il, grids);
+    if (il > lmin && icount < nsub) ensure_ref_rules(a, grids);
     return RH_OK;
 }
 
@@ -727,7 +730,7 @@ int make_grid_fine(rh_amr* a, int il) {
     RH_AHIP(a, hipMemcpyAsync(a->d_cells, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice,
                               a->ctx->stream));
     RH_AHIP(a, rh::launch_new_octs(mesh_dev(a->ctx), n, a->d_cells, gas(a->ctx, a->gamma), a->hp.interpol_var,
-                                   a->ctx->stream));
+                                   a->d_father, a->d_nbor, a->ctx->stream));
     RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));  // idx goes out of scope
     tree_changed(a);
     return RH_OK;

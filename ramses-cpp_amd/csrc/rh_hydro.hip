@@ -1096,11 +1096,16 @@ __global__ __launch_bounds__(64) void k_interpol(int n_octs, const double* __res
 // cell itself; no reflective boundaries exist), cells[8o+7] the new grid.
 // The eight children get the prolongated state in uold and unew.
 __global__ __launch_bounds__(64) void k_new_octs(MeshDev m, int n, const int* __restrict__ cells, GasParams gp,
-                                                 int interpol_var) {
+                                                 int interpol_var, int* __restrict__ father, int* __restrict__ nbor) {
     const int o = blockIdx.x * 64 + threadIdx.x;
     if (o >= n) return;
     const int* c = cells + (size_t)o * 8;
     const int grid = c[7];
+    // The new oct's links, as the host tree holds them: father = the refined
+    // cell, nbor[f] = its six neighbour cells.
+    father[grid - 1] = c[0];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nbor[(size_t)f * m.ngridmax + grid - 1] = c[1 + f];
     interpol_oct(
         [&](int i, int iv) {
             const int id = c[i] > 0 ? c[i] : c[0];
@@ -1378,9 +1383,10 @@ hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams&
 }
 
 hipError_t launch_new_octs(const MeshDev& m, int n, const int* d_cells, const GasParams& gp, int interpol_var,
-                           hipStream_t stream) {
+                           int* d_father, int* d_nbor, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_new_octs, dim3((n + 63) / 64), dim3(64), 0, stream, m, n, d_cells, gp, interpol_var);
+    hipLaunchKernelGGL(k_new_octs, dim3((n + 63) / 64), dim3(64), 0, stream, m, n, d_cells, gp, interpol_var,
+                       d_father, d_nbor);
     return hipGetLastError();
 }
 

@@ -85,11 +85,12 @@ hipError_t launch_fill_keys(unsigned long long* keys, int n, double value, hipSt
 // dt[i] <- the double encoded by keys[i], i < n.
 hipError_t launch_keys_to_dt(const unsigned long long* keys, int n, double* dt, hipStream_t stream);
 // Adaptive run: prolongation into new octs (d_cells: 8 ints per new oct,
-// the refined cell, its six neighbours, the new grid), restriction of a list
-// into its fathers, and the gradient refinement criteria of one list
-// (d_out[8*pos + ic-1]).
+// the refined cell, its six neighbours, the new grid; the same values go to
+// the new grid's entries of d_father [ngridmax] and d_nbor [6*ngridmax]),
+// restriction of a list into its fathers, and the gradient refinement
+// criteria of one list (d_out[8*pos + ic-1]).
 hipError_t launch_new_octs(const MeshDev& m, int n, const int* d_cells, const GasParams& gp, int interpol_var,
-                           hipStream_t stream);
+                           int* d_father, int* d_nbor, hipStream_t stream);
 hipError_t launch_restrict(const MeshDev& m, const LevelDev& fine, const int* d_father, hipStream_t stream);
 hipError_t launch_grad_flags(const MeshDev& m, const LevelDev& lv, const int* d_nbor, double ed, double ep,
                              double ev, double gamma, unsigned char* d_out, hipStream_t stream);
