@@ -56,6 +56,8 @@ struct rh_amr {
     // Simulation (Simulation.hpp).
     int levelmin = 1;
     std::vector<int> nsubcycle;
+    // Host copies of the device time-step table (d_times), refreshed by
+    // sync_times before the run's time is reported.
     std::vector<double> dtnew, dtold;
     double t = 0.0;
     int nstep = 0;
@@ -68,6 +70,7 @@ struct rh_amr {
     int* d_cells = nullptr;
     int ncells_cap = 0;
     unsigned char* d_flags = nullptr;
+    double* d_times = nullptr;  // rh::AMR_T_SIZE doubles
     bool topo_dirty = true;
     long long epoch = 1;
     rh_mesh mesh{};
@@ -156,6 +159,21 @@ int upload_topology(rh_amr* a) {
     RH_AHIP(a, hipMemcpyAsync(a->d_nbor, a->nbor.data(), (size_t)6 * a->ngridmax * sizeof(int),
                               hipMemcpyHostToDevice, a->ctx->stream));
     a->topo_dirty = false;
+    return RH_OK;
+}
+
+// The device time-step table into the host copies (waits for the stream).
+int sync_times(rh_amr* a) {
+    std::vector<double> T(rh::AMR_T_SIZE);
+    RH_AHIP(a, hipSetDevice(a->hp.device));
+    RH_AHIP(a, hipMemcpyAsync(T.data(), a->d_times, T.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              a->ctx->stream));
+    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
+    for (int l = 0; l < rh::AMR_T_LEVELS && l < (int)a->dtnew.size(); ++l) {
+        a->dtnew[l] = T[rh::AMR_T_DTNEW + l];
+        a->dtold[l] = T[rh::AMR_T_DTOLD + l];
+    }
+    a->t = T[rh::AMR_T_TIME];
     return RH_OK;
 }
 
@@ -875,17 +893,13 @@ int amr_step(rh_amr* a, int il, int icount) {
         for (int i = a->nlevelmax; i >= il; --i) remove_grid_fine(a, i + 1);
     }
     const double dx = a->boxlen / (double)(a->nx * (1 << il));
-    a->dtold[il] = a->dtnew[il];
-    double dtc = 0.0;
+    // dtold/dtnew/t live in d_times: the Courant minimum, the subcycling
+    // limit and the time update are stream-ordered on the device.
+    const hipStream_t stream = a->ctx->stream;
     {
         Phase ph{a, PH_HYDRO};
-        RH_ACHK(rh_courant(a->ctx, il, dx, a->gamma, a->ap.courant_factor, &dtc));
-    }
-    a->dtnew[il] = dtc;
-    if (il > a->levelmin) a->dtnew[il] = std::min(a->dtnew[il], a->dtnew[il - 1] / (double)a->nsubcycle[il - 1]);
-    const double dt = a->dtnew[il];
-    {
-        Phase ph{a, PH_HYDRO};
+        RH_ACHK(courant_dev(a->ctx, il, dx, a->gamma, a->ap.courant_factor, a->d_times + rh::AMR_T_DTC));
+        RH_AHIP(a, rh::launch_amr_times(a->d_times, 0, il, a->levelmin, (double)a->nsubcycle[il - 1], 0.0, stream));
         RH_ACHK(rh_set_unew(a->ctx, il));
     }
     const int nsub = il < (int)a->nsubcycle.size() ? a->nsubcycle[il] : 1;
@@ -893,13 +907,11 @@ int amr_step(rh_amr* a, int il, int icount) {
         if (count(a, il + 1) > 0) {
             for (int i = 1; i <= nsub; ++i) RH_ACHK(amr_step(a, il + 1, i));
         } else {
-            a->dtold[il + 1] = a->dtnew[il] / (double)nsub;
-            a->dtnew[il + 1] = a->dtnew[il] / (double)nsub;
-            a->t += dt;
+            RH_AHIP(a, rh::launch_amr_times(a->d_times, 1, il, a->levelmin, 0.0, (double)nsub, stream));
             a->nstep++;
         }
     } else {
-        a->t += dt;
+        RH_AHIP(a, rh::launch_amr_times(a->d_times, 2, il, a->levelmin, 0.0, 0.0, stream));
         a->nstep++;
     }
     {
@@ -907,7 +919,7 @@ int amr_step(rh_amr* a, int il, int icount) {
         long long nleaf = 0;
         RH_ACHK(rh_level_leaf_cells(a->ctx, il, &nleaf));
         a->cell_updates += nleaf;
-        RH_ACHK(rh_godunov_fine(a->ctx, il, dt, dx));
+        RH_ACHK(godunov_dev(a->ctx, il, a->d_times + rh::AMR_T_USED + il, dx));
         RH_ACHK(rh_set_uold(a->ctx, il));
     }
     if (il < a->nlevelmax) {
@@ -938,7 +950,7 @@ int step_coarse(rh_amr* a) {
 void destroy(rh_amr* a) {
     if (a->ctx) {
         (void)hipSetDevice(a->hp.device);
-        for (void* p : {(void*)a->d_father, (void*)a->d_nbor, (void*)a->d_cells, (void*)a->d_flags})
+        for (void* p : {(void*)a->d_father, (void*)a->d_nbor, (void*)a->d_cells, (void*)a->d_flags, (void*)a->d_times})
             if (p) (void)hipFree(p);
         rh_destroy(a->ctx);
     }
@@ -989,6 +1001,11 @@ int rh_amr_create(const rh_params* hydro, const rh_amr_params* ap, rh_amr** out)
         hipError_t e = hipMalloc(&a->d_father, (size_t)a->ngridmax * sizeof(int));
         if (e == hipSuccess) e = hipMalloc(&a->d_nbor, (size_t)6 * a->ngridmax * sizeof(int));
         if (e == hipSuccess) e = hipMalloc(&a->d_flags, (size_t)8 * a->ngridmax);
+        if (e == hipSuccess) e = hipMalloc(&a->d_times, (size_t)rh::AMR_T_SIZE * sizeof(double));
+        if (e == hipSuccess) {
+            const std::vector<double> zero(rh::AMR_T_SIZE, 0.0);  // dtnew = dtold = 0, t = 0
+            e = hipMemcpy(a->d_times, zero.data(), zero.size() * sizeof(double), hipMemcpyHostToDevice);
+        }
         if (e != hipSuccess) st = afail(a, RH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
     if (st == RH_OK) st = initialize(a);
@@ -1026,6 +1043,7 @@ int rh_amr_mesh(rh_amr* a, rh_mesh* mesh, const int** numbl, const double** xg) 
 
 int rh_amr_stats(rh_amr* a, double* t, int* nstep, long long* cell_updates) {
     if (!a) return RH_ERR_ARG;
+    RH_ACHK(sync_times(a));
     if (t) *t = a->t;
     if (nstep) *nstep = a->nstep;
     if (cell_updates) *cell_updates = a->cell_updates;
@@ -1044,6 +1062,7 @@ int rh_amr_dump(rh_amr* a, const rh_snapshot_info* info, int iout, const char* r
         in.units_length = in.units_density = in.units_time = 1.0;
     }
     // What Simulation::dump_snapshot takes from the run (Simulation.cpp:683-692).
+    RH_ACHK(sync_times(a));
     in.t = a->t;
     in.nstep = in.nstep_coarse = a->nstep;
     in.iout = iout;

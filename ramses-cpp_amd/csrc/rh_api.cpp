@@ -260,6 +260,30 @@ int do_godunov(rh_ctx* c, const rh::LevelDev* lv, const double* d_dt, double dx)
     return RH_OK;
 }
 
+// godunov_fine and the Courant step with the time step in device memory
+// (the adaptive driver's device-side bookkeeping): the same launches as
+// rh_godunov_fine / rh_courant, without the host round trip.  Single mesh in
+// device-resident mode only.
+int courant_dev(rh_ctx* c, int ilevel, double dx, double gamma, double courant_factor, double* d_out) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (c->mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "device time steps need a device-resident mesh");
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    return do_courant(c, lv, dx, gamma, courant_factor, d_out);
+}
+
+int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (c->mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "device time steps need a device-resident mesh");
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    const rh::LevelDev* lv;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    return do_godunov(c, lv, d_dt, dx);
+}
+
 }  // namespace rhi
 
 using namespace rhi;

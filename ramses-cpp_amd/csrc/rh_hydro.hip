@@ -1119,6 +1119,39 @@ __global__ __launch_bounds__(64) void k_new_octs(MeshDev m, int n, const int* __
         gp, interpol_var);
 }
 
+// Time-step bookkeeping of the adaptive run's subcycling (Simulation.cpp:
+// 522-635) on the device, so that no fine step waits on the host for its
+// Courant step.  One thread; T as laid out by AMR_T_* in rh_kernels.h.
+// op 0: after the level's Courant step (its minimum in T[AMR_T_DTC]):
+//   dtold[il] = dtnew[il]; dtnew[il] = min(dtc, dtnew[il-1]/nsub_prev) for
+//   il > lmin, else dtc (std::min's operand order); used[il] = dtnew[il].
+// op 1: a level without finer octs: dtold/dtnew[il+1] = dtnew[il]/nsub, then
+//   t += used[il].  op 2: the finest level: t += used[il].
+__global__ __launch_bounds__(64) void k_amr_times(double* T, int op, int il, int lmin, double nsub_prev,
+                                                  double nsub) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double* dtnew = T + AMR_T_DTNEW;
+    double* dtold = T + AMR_T_DTOLD;
+    double* used = T + AMR_T_USED;
+    if (op == 0) {
+        dtold[il] = dtnew[il];
+        double v = T[AMR_T_DTC];
+        if (il > lmin) {
+            const double w = dtnew[il - 1] / nsub_prev;
+            v = (w < v) ? w : v;
+        }
+        dtnew[il] = v;
+        used[il] = v;
+    } else {
+        if (op == 1) {
+            const double w = dtnew[il] / nsub;
+            dtold[il + 1] = w;
+            dtnew[il + 1] = w;
+        }
+        T[AMR_T_TIME] = T[AMR_T_TIME] + used[il];
+    }
+}
+
 // restrict_fine (TreeUpdater.cpp:195-215): the father cell of every oct of
 // the finer list gets the mean of its eight children's uold.
 __global__ __launch_bounds__(256) void k_restrict(MeshDev m, LevelDev fine, const int* __restrict__ father) {
@@ -1387,6 +1420,13 @@ hipError_t launch_new_octs(const MeshDev& m, int n, const int* d_cells, const Ga
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_new_octs, dim3((n + 63) / 64), dim3(64), 0, stream, m, n, d_cells, gp, interpol_var,
                        d_father, d_nbor);
+    return hipGetLastError();
+}
+
+hipError_t launch_amr_times(double* d_T, int op, int il, int lmin, double nsub_prev, double nsub,
+                            hipStream_t stream) {
+    if (il < 1 || il + 1 >= AMR_T_LEVELS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_amr_times, dim3(1), dim3(64), 0, stream, d_T, op, il, lmin, nsub_prev, nsub);
     return hipGetLastError();
 }
 

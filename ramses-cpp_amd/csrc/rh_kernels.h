@@ -92,6 +92,21 @@ hipError_t launch_keys_to_dt(const unsigned long long* keys, int n, double* dt, 
 hipError_t launch_new_octs(const MeshDev& m, int n, const int* d_cells, const GasParams& gp, int interpol_var,
                            int* d_father, int* d_nbor, hipStream_t stream);
 hipError_t launch_restrict(const MeshDev& m, const LevelDev& fine, const int* d_father, hipStream_t stream);
+// Adaptive run's time-step table on the device (doubles): dtnew, dtold and
+// the dt each level's last step used, AMR_T_LEVELS entries each, then the
+// time t and the latest Courant minimum.
+constexpr int AMR_T_LEVELS = 33;
+constexpr int AMR_T_DTNEW = 0;
+constexpr int AMR_T_DTOLD = AMR_T_LEVELS;
+constexpr int AMR_T_USED = 2 * AMR_T_LEVELS;
+constexpr int AMR_T_TIME = 3 * AMR_T_LEVELS;
+constexpr int AMR_T_DTC = 3 * AMR_T_LEVELS + 1;
+constexpr int AMR_T_SIZE = 3 * AMR_T_LEVELS + 2;
+// One bookkeeping step (op 0: after a Courant step; 1: a level without finer
+// octs; 2: the finest level), as k_amr_times in rh_hydro.hip describes.
+hipError_t launch_amr_times(double* d_T, int op, int il, int lmin, double nsub_prev, double nsub,
+                            hipStream_t stream);
+
 hipError_t launch_grad_flags(const MeshDev& m, const LevelDev& lv, const int* d_nbor, double ed, double ep,
                              double ev, double gamma, unsigned char* d_out, hipStream_t stream);
 hipError_t launch_interpol(int n_octs, const double* d_u1, double* d_u2, const GasParams& gp, int interpol_var,
