@@ -118,23 +118,36 @@ int build_topo_index(const rh_mesh& m, TopoIndex& ti, std::string& err) {
             ig = m.next[ig - 1];
         }
     }
-    // Father-chain depth, as HydroSolver.cpp:22-38 computes cell_levels.
+    // Father-chain depth, as HydroSolver.cpp:22-38 computes cell_levels: the
+    // number of grids on the chain up to the first whose father is a coarse
+    // cell (or 0).  Each chain is walked only up to a grid whose depth is
+    // already known, so the whole pass is linear in the grids.  Grids with
+    // father 0 keep depth 0 (the reference never asks for theirs).
+    std::vector<int> chain;
     for (int ig = 1; ig <= m.ngridmax; ++ig) {
-        if (m.father[ig - 1] <= 0) continue;
-        int level = 1;
-        int cur = ig;
-        int guard = 0;
-        while (cur > 0) {
-            int fc = m.father[cur - 1];
+        if (m.father[ig - 1] <= 0 || ti.depth[ig] != 0) continue;
+        chain.clear();
+        int cur = ig, base = 0;
+        for (;;) {
+            chain.push_back(cur);
+            const int fc = m.father[cur - 1];
             if (fc <= m.ncoarse) break;
             cur = ((fc - m.ncoarse - 1) % m.ngridmax) + 1;
-            level++;
-            if (++guard > 64) {
+            if (ti.depth[cur] != 0) {
+                base = ti.depth[cur];
+                break;
+            }
+            if ((int)chain.size() > 64) {
                 err = "father chain longer than 64 from grid " + std::to_string(ig);
                 return RH_ERR_TOPOLOGY;
             }
         }
-        ti.depth[ig] = level;
+        if (base + (int)chain.size() > 65) {
+            err = "father chain longer than 64 from grid " + std::to_string(ig);
+            return RH_ERR_TOPOLOGY;
+        }
+        for (int k = (int)chain.size() - 1, d = base + 1; k >= 0; --k, ++d)
+            if (m.father[chain[k] - 1] > 0) ti.depth[chain[k]] = d;
     }
     ti.epoch = m.mesh_epoch;
     return RH_OK;
