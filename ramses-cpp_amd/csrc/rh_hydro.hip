@@ -949,13 +949,16 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
     ordered_apply<O_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn);
 }
 
-// Reflux into coarser leaf cells, one wave per target cell, contributions in
-// processing order.  A coarse cell next to a finer region can receive dozens
-// of contributions; the wave loads up to 64 of them at once (slot index,
-// then the five values) into LDS and lane 0 subtracts them in order, so a
-// target costs two memory round trips per 64 contributions instead of a
-// chain of dependent loads.
-__global__ __launch_bounds__(64) void k_coarse_reflux(MeshDev m, LevelDev lv) {
+// Reflux into coarser leaf cells, contributions in processing order.  Two
+// forms with identical results (each target's contributions subtracted one
+// after another in list order), chosen by the number of target cells:
+//  - k_coarse_reflux_wave, one wave per target: the wave loads up to 64
+//    contributions at once (slot index, then the five values) into LDS and
+//    lane 0 subtracts them in order.  Latency-optimal for the few hundred to
+//    few thousand targets of a small level, whose lists run to dozens.
+//  - k_coarse_reflux, eight lanes per target (below): for levels with many
+//    targets, whose lists are short.
+__global__ __launch_bounds__(64) void k_coarse_reflux_wave(MeshDev m, LevelDev lv) {
     __shared__ double s_v[64][5];
     const int t = blockIdx.x;
     const int lane = threadIdx.x;
@@ -983,6 +986,64 @@ __global__ __launch_bounds__(64) void k_coarse_reflux(MeshDev m, LevelDev lv) {
         __syncthreads();
     }
     if (lane == 0) {
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
+    }
+}
+
+// Reflux, eight lanes per target cell (32 targets per 256-thread block).  The eight lanes
+// of a group load the target's next 32 contributions (slot index, then the
+// five values; four per lane, all loads independent) at once, and the
+// group's first lane subtracts them in order, taking each from its lane by
+// shuffle.  A coarse cell next to a finer region can receive dozens of
+// contributions: it costs one round trip per 32 of them instead of a chain of
+// dependent loads, while the many targets with a few contributions each still
+// fill the lanes.
+constexpr int RF_LANES = 8;
+constexpr int RF_PER_LANE = 4;
+constexpr int RF_CHUNK = RF_LANES * RF_PER_LANE;
+constexpr int RF_BLOCK = 256;
+// Levels with fewer targets use the wave-per-target form.
+constexpr int RF_GROUP_MIN_TARGETS = 4096;
+
+__global__ __launch_bounds__(RF_BLOCK) void k_coarse_reflux(MeshDev m, LevelDev lv) {
+    const long long t = ((long long)blockIdx.x * RF_BLOCK + threadIdx.x) / RF_LANES;
+    const int j = threadIdx.x & (RF_LANES - 1);
+    // The group's first lane within the wave; a group never straddles waves.
+    const int lead = (threadIdx.x & 63) & ~(RF_LANES - 1);
+    // Whole groups leave together, so every shuffle below reads active lanes.
+    if (t >= lv.ntgt) return;
+    const size_t c = (size_t)lv.tgt_cell[t] - 1;
+    double u[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (j == 0) {
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) u[iv] = m.unew[(size_t)iv * m.ncell + c];
+    }
+    const int beg = lv.tgt_off[t], end = lv.tgt_off[t + 1];
+    for (int k = beg; k < end; k += RF_CHUNK) {
+        const int n = min(RF_CHUNK, end - k);
+        // Contribution k + r*RF_LANES + j sits in lane j, register row r.
+        double v[RF_PER_LANE][5];
+#pragma unroll
+        for (int r = 0; r < RF_PER_LANE; ++r) {
+            const int e = r * RF_LANES + j;
+            const size_t slot = e < n ? (size_t)lv.tgt_slot[k + e] : 0;
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) v[r][iv] = e < n ? lv.slot_r[slot * 5 + iv] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < RF_PER_LANE; ++r) {
+            const int nr = min(RF_LANES, n - r * RF_LANES);  // group-uniform
+            for (int i = 0; i < nr; ++i) {
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) {
+                    const double w = __shfl(v[r][iv], lead + i, 64);
+                    u[iv] = u[iv] - w;
+                }
+            }
+        }
+    }
+    if (j == 0) {
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
     }
@@ -1411,7 +1472,13 @@ hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams&
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || lv.ntgt == 0) return e;
-    hipLaunchKernelGGL(k_coarse_reflux, dim3(lv.ntgt), dim3(64), 0, stream, m, lv);
+    if (lv.ntgt < RF_GROUP_MIN_TARGETS) {
+        hipLaunchKernelGGL(k_coarse_reflux_wave, dim3(lv.ntgt), dim3(64), 0, stream, m, lv);
+    } else {
+        const size_t rf_threads = (size_t)lv.ntgt * RF_LANES;
+        hipLaunchKernelGGL(k_coarse_reflux, dim3((unsigned)((rf_threads + RF_BLOCK - 1) / RF_BLOCK)),
+                           dim3(RF_BLOCK), 0, stream, m, lv);
+    }
     return hipGetLastError();
 }
 

@@ -112,3 +112,36 @@ def test_rebind_after_refinement():
         api.godunov_fine(5, 1e-3, 1 / 32)
     assert_same(m.unew, mo.unew, "unew after rebind")
     s.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_checkerboard_refinement_many_reflux_targets(variant):
+    """Every other cell of a 32^3 level refined in a 3D checkerboard: each
+    fine oct is surrounded by coarser leaf cells on all six faces, and the
+    16384 unrefined cells are all reflux targets with 24 contributions each
+    -- enough targets for the eight-lanes-per-target reflux form (the smaller
+    meshes above use the wave-per-target form)."""
+    from amr_meshes import cell_centres
+
+    base = 6
+    ng = int(rhydro.lib().rh_uniform_ngrid(1, 1, 1, base))
+    m0 = rhydro.uniform_mesh(base, ngridmax=ng + 16384, nlevelmax=base + 1)
+    rhydro.init_lognormal(m0, base, 13, 1.4, 0.3, 0.3)
+    cells, cen = cell_centres(m0, base)
+    ijk = np.floor(cen * 2 ** (base - 1)).astype(np.int64)
+    chosen = cells[(ijk.sum(1) % 2) == 0]
+    assert chosen.size == 16384
+    oracle.refine(m0, oracle.params(1.4, 1, 2), base, chosen)
+    m0.unew[:] = m0.uold
+    levels = [base + 1, base]
+    dts = [1e-3, 2e-3]
+    print(f"case amr checkerboard variant={variant} octs={m0.numbl.tolist()}", flush=True)
+    mg, mo = clone(m0), clone(m0)
+    s = rhydro.Solver(gamma=1.4, slope_type=1, riemann="hllc", variant=variant)
+    s.bind(mg)
+    _sequence(s, levels, dts)
+    s.sync_to_host()
+    _sequence(_OracleApi(mo, oracle.params(1.4, 1, 2)), levels, dts)
+    assert_same(mg.unew, mo.unew, "unew")
+    assert_same(mg.uold, mo.uold, "uold")
+    s.close()
