@@ -61,7 +61,9 @@ int rh_amr_step(rh_amr* run, int nsteps);
 int rh_amr_mesh(rh_amr* run, rh_mesh* mesh, const int** numbl, const double** xg);
 
 /* Time, fine-step count, and leaf cells updated by godunov_fine since
- * creation (the cell-update count of the benchmark metric). */
+ * creation (the cell-update count of the benchmark metric).  The run keeps
+ * its time and time steps in device memory: this call reads them back and so
+ * waits for the run's queued work. */
 int rh_amr_stats(rh_amr* run, double* t, int* nstep, long long* cell_updates);
 
 /* The run's hydro context (kernel timers, variant), owned by the run. */
