@@ -38,6 +38,16 @@ PROFILED_TRAFFIC = {
     (9, "hllc", 1, 1, "mixed"): 5.164e9,
 }
 
+# VALU utilisation of the same launches from an SQ/GRBM counter run
+# (SQ_ACTIVE_INST_VALU x 4 / SIMDs / active cycles per XCD) and VALU
+# instructions per wave (= per cell: 64 cells per wave) from SQ_INSTS_VALU /
+# SQ_WAVES, same key.
+PROFILED_VALU = {
+    (9, "hllc", 1, 1, "fp64"): {"busy": 0.73, "insts_per_cell": 5739},
+    (9, "hllc", 1, 1, "mixed"): {"busy": 0.67, "insts_per_cell": 4004},
+}
+PROFILE_NOTE = "profiles/godunov_level9_256cubed_hllc_minmod.md (rocprofv3 counter runs of this script)"
+
 
 def cpu_model() -> str:
     try:
@@ -418,6 +428,8 @@ def main() -> int:
             "roofline_frac": mixed["achieved"] / HBM_PEAK_GBS,
             "traffic": PROFILED_TRAFFIC.get((level, args.riemann, args.slope, mixed["autotune"]["chosen_variant"],
                                              "mixed")),
+            "valu": PROFILED_VALU.get((level, args.riemann, args.slope, mixed["autotune"]["chosen_variant"],
+                                       "mixed")),
             "speedup_vs_fp64": fp64["wall"] / mixed["wall"],
             "error_vs_fp64_one_step": err1,
             "error_vs_fp64_after_steps": errn,
@@ -514,6 +526,8 @@ def main() -> int:
             "slope_type": args.slope,
             "kernel_ms": fp64["kernel_ms"],
             "autotune": fp64["autotune"],
+            "valu": PROFILED_VALU.get((level, args.riemann, args.slope, fp64["autotune"]["chosen_variant"], "fp64")),
+            "counters_from": PROFILE_NOTE,
             "mixed": mixed_info,
             "variants": variants,
             "ensemble": ens_info,
