@@ -31,11 +31,13 @@ BYTES_PER_CELL_UPDATE = 80  # read uold (5 x FP64) + write unew (5 x FP64)
 # HBM bytes per launch of the Godunov kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE and --pmc WRITE_SIZE runs of this script (each counter scaled by
 # what it reads for k_set_unew, a copy of known size, in the same run); keyed
-# by (level, riemann, slope_type, kernel variant, form).  Counters cannot be
-# read while timing, so only profiled workloads report a figure.
+# by (level, riemann, slope_type, kernel, form), kernel "oct" for the
+# oct-in-lanes variants (profiled at 16 octs per workgroup; the 8-oct variant
+# makes the same accesses) and "cell" for the per-cell one.  Counters cannot
+# be read while timing, so only profiled workloads report a figure.
 PROFILED_TRAFFIC = {
-    (9, "hllc", 1, 1, "fp64"): 5.355e9,
-    (9, "hllc", 1, 1, "mixed"): 5.164e9,
+    (9, "hllc", 1, "oct", "fp64"): 5.355e9,
+    (9, "hllc", 1, "oct", "mixed"): 5.164e9,
 }
 
 # VALU utilisation of the same launches from an SQ/GRBM counter run
@@ -43,9 +45,13 @@ PROFILED_TRAFFIC = {
 # instructions per wave (= per cell: 64 cells per wave) from SQ_INSTS_VALU /
 # SQ_WAVES, same key.
 PROFILED_VALU = {
-    (9, "hllc", 1, 1, "fp64"): {"busy": 0.73, "insts_per_cell": 5739},
-    (9, "hllc", 1, 1, "mixed"): {"busy": 0.67, "insts_per_cell": 4004},
+    (9, "hllc", 1, "oct", "fp64"): {"busy": 0.73, "insts_per_cell": 5739},
+    (9, "hllc", 1, "oct", "mixed"): {"busy": 0.67, "insts_per_cell": 4004},
 }
+
+
+def kernel_family(variant: int) -> str:
+    return "cell" if variant == 0 else "oct"
 PROFILE_NOTE = "profiles/godunov_level9_256cubed_hllc_minmod.md (rocprofv3 counter runs of this script)"
 
 
@@ -426,9 +432,9 @@ def main() -> int:
             "kernel_ms": mixed["kernel_ms"],
             "autotune": mixed["autotune"],
             "roofline_frac": mixed["achieved"] / HBM_PEAK_GBS,
-            "traffic": PROFILED_TRAFFIC.get((level, args.riemann, args.slope, mixed["autotune"]["chosen_variant"],
+            "traffic": PROFILED_TRAFFIC.get((level, args.riemann, args.slope, kernel_family(mixed["autotune"]["chosen_variant"]),
                                              "mixed")),
-            "valu": PROFILED_VALU.get((level, args.riemann, args.slope, mixed["autotune"]["chosen_variant"],
+            "valu": PROFILED_VALU.get((level, args.riemann, args.slope, kernel_family(mixed["autotune"]["chosen_variant"]),
                                        "mixed")),
             "speedup_vs_fp64": fp64["wall"] / mixed["wall"],
             "error_vs_fp64_one_step": err1,
@@ -526,7 +532,7 @@ def main() -> int:
             "slope_type": args.slope,
             "kernel_ms": fp64["kernel_ms"],
             "autotune": fp64["autotune"],
-            "valu": PROFILED_VALU.get((level, args.riemann, args.slope, fp64["autotune"]["chosen_variant"], "fp64")),
+            "valu": PROFILED_VALU.get((level, args.riemann, args.slope, kernel_family(fp64["autotune"]["chosen_variant"]), "fp64")),
             "counters_from": PROFILE_NOTE,
             "mixed": mixed_info,
             "variants": variants,
@@ -541,7 +547,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": PROFILED_TRAFFIC.get((level, args.riemann, args.slope, fp64["autotune"]["chosen_variant"], "fp64")),
+            "traffic": PROFILED_TRAFFIC.get((level, args.riemann, args.slope, kernel_family(fp64["autotune"]["chosen_variant"]), "fp64")),
         },
         "cpu_baseline": cpu,
     }

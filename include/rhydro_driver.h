@@ -76,15 +76,18 @@ int rh_timers_reset(rh_ctx* ctx);
 /* Godunov kernel variant of a context: 0 = one thread per cell with the
  * octs of a workgroup strided across lanes; 1 (default) = the eight cells of
  * an oct in adjacent lanes, sharing sibling states and fluxes by cross-lane
- * shuffles.  Both produce identical results. */
+ * shuffles, 16 octs per workgroup; 2 = the same with 8 octs per workgroup.
+ * All produce identical results. */
+#define RH_KERNEL_VARIANTS 3
 int rh_set_kernel_variant(rh_ctx* ctx, int variant);
 int rh_get_kernel_variant(rh_ctx* ctx, int* variant);
 
 /* Times each Godunov kernel variant on level `ilevel` of the bound mesh
  * (reps timed launches after one untimed launch, HIP events on the
  * context's stream), keeps the fastest for the context and reports it in
- * *best_variant and the average ms per launch in ms_per_variant[2] (either
- * pointer may be NULL).  unew is restored after every trial. */
+ * *best_variant and the average ms per launch in
+ * ms_per_variant[RH_KERNEL_VARIANTS] (either pointer may be NULL).  unew is
+ * restored after every trial. */
 int rh_autotune(rh_ctx* ctx, int ilevel, double dt, double dx, int reps, int* best_variant, double* ms_per_variant);
 
 /* Leaf cells of a level in the bound mesh (cell-updates per godunov call). */

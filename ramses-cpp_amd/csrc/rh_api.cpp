@@ -543,7 +543,7 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
 
 int rh_set_kernel_variant(rh_ctx* c, int variant) {
     if (!c) return RH_ERR_ARG;
-    if (variant != 0 && variant != 1) return fail(c, RH_ERR_ARG, "unknown kernel variant");
+    if (variant < 0 || variant >= RH_KERNEL_VARIANTS) return fail(c, RH_ERR_ARG, "unknown kernel variant");
     c->variant = variant;
     return RH_OK;
 }
@@ -570,7 +570,8 @@ int rh_autotune(rh_ctx* c, int ilevel, double dt, double dx, int reps, int* best
     int best = keep;
     float best_ms = 0.f;
     int rc = RH_OK;
-    for (int v = 0; v < 2 && rc == RH_OK; ++v) {
+    static_assert(RH_KERNEL_VARIANTS == rh::KERNEL_VARIANTS, "variant count of the C-ABI and the kernels");
+    for (int v = 0; v < RH_KERNEL_VARIANTS && rc == RH_OK; ++v) {
         c->variant = v;
         float total = 0.f;
         for (int r = 0; r <= reps && rc == RH_OK; ++r) {  // first launch untimed

@@ -710,23 +710,26 @@ template <typename T> RH_DEV void sel5(bool c, const T a[5], const T b[5], T out
     for (int i = 0; i < 5; ++i) out[i] = c ? a[i] : b[i];
 }
 
-constexpr int O_OCTS = 16;
-constexpr int O_BLOCK = O_OCTS * 8;
+// Octs per workgroup: 16 (two waves, kernel variant 1) or 8 (one wave,
+// variant 2); rh_autotune times both.
+template <int OCTS> constexpr int oct_block() { return OCTS * 8; }
 
 // Occupancy: both forms run two waves per SIMD (at most 256 VGPRs), with
-// 36 KB of LDS per workgroup for the incoming contributions and their keys.
-// The registers above what the arithmetic needs hold the next dimension's
-// neighbour states, gathered ahead of their use.
-template <int RIEMANN, bool MIXED, bool EXACT>
-__global__ __launch_bounds__(O_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8)))
+// 2.25 KB of LDS per oct (36 KB per 16-oct workgroup) for the incoming
+// contributions and their keys.  The registers above what the arithmetic
+// needs hold the next dimension's neighbour states, gathered ahead of their
+// use.
+template <int RIEMANN, bool MIXED, bool EXACT, int OCTS>
+__global__ __launch_bounds__(OCTS * 8) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                                                           double dx, double inv_dx) {
     using TC = typename std::conditional<MIXED, float, double>::type;
-    __shared__ double s_r[30][O_BLOCK];
-    __shared__ long long s_k[6][O_BLOCK];
+    constexpr int NT = oct_block<OCTS>();
+    __shared__ double s_r[30][NT];
+    __shared__ long long s_k[6][NT];
     const int t = threadIdx.x;
     const int ic0 = t & 7;
-    const int praw = blockIdx.x * O_OCTS + (t >> 3);
+    const int praw = blockIdx.x * OCTS + (t >> 3);
     const int noct = lv.noct;
     // Lanes past the end of the list work on the last oct (valid memory) and
     // write nothing, so every lane of an oct group takes part in the shuffles.
@@ -946,7 +949,7 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         s_k[slotX][t] = Kin;
     }
     if (!leaf) return;
-    ordered_apply<O_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn);
+    ordered_apply<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn);
 }
 
 // Reflux into coarser leaf cells, contributions in processing order.  Two
@@ -1434,22 +1437,27 @@ hipError_t launch_keys_to_dt(const unsigned long long* keys, int n, double* dt, 
 namespace {
 
 template <int R, bool MIXED, bool EXACT>
-void launch_form(bool oct, dim3 grid, dim3 block, const MeshDev& m, const LevelDev& lv, const GasParams& gp,
-                 const double* d_dt, double dx, double inv_dx, hipStream_t stream) {
-    if (oct)
-        hipLaunchKernelGGL((k_godunov_oct<R, MIXED, EXACT>), grid, block, 0, stream, m, lv, gp, d_dt, dx, inv_dx);
+void launch_form(int variant, const MeshDev& m, const LevelDev& lv, const GasParams& gp, const double* d_dt,
+                 double dx, double inv_dx, hipStream_t stream) {
+    if (variant == 1)
+        hipLaunchKernelGGL((k_godunov_oct<R, MIXED, EXACT, 16>), dim3((lv.noct + 15) / 16), dim3(oct_block<16>()), 0,
+                           stream, m, lv, gp, d_dt, dx, inv_dx);
+    else if (variant == 2)
+        hipLaunchKernelGGL((k_godunov_oct<R, MIXED, EXACT, 8>), dim3((lv.noct + 7) / 8), dim3(oct_block<8>()), 0,
+                           stream, m, lv, gp, d_dt, dx, inv_dx);
     else
-        hipLaunchKernelGGL((k_godunov<R, MIXED, EXACT>), grid, block, 0, stream, m, lv, gp, d_dt, dx, inv_dx);
+        hipLaunchKernelGGL((k_godunov<R, MIXED, EXACT>), dim3((lv.noct + G_OCTS - 1) / G_OCTS), dim3(G_BLOCK), 0,
+                           stream, m, lv, gp, d_dt, dx, inv_dx);
 }
 
 // The FP32 half of the mixed form divides through dv() whatever dx is, so
 // only the FP64 form has an exact-reciprocal instantiation.
 template <int R>
-void launch_riemann(bool oct, bool mixed, bool exact, dim3 grid, dim3 block, const MeshDev& m, const LevelDev& lv,
-                    const GasParams& gp, const double* d_dt, double dx, double inv_dx, hipStream_t stream) {
-    if (mixed) launch_form<R, true, false>(oct, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream);
-    else if (exact) launch_form<R, false, true>(oct, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream);
-    else launch_form<R, false, false>(oct, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream);
+void launch_riemann(int variant, bool mixed, bool exact, const MeshDev& m, const LevelDev& lv, const GasParams& gp,
+                    const double* d_dt, double dx, double inv_dx, hipStream_t stream) {
+    if (mixed) launch_form<R, true, false>(variant, m, lv, gp, d_dt, dx, inv_dx, stream);
+    else if (exact) launch_form<R, false, true>(variant, m, lv, gp, d_dt, dx, inv_dx, stream);
+    else launch_form<R, false, false>(variant, m, lv, gp, d_dt, dx, inv_dx, stream);
 }
 
 }  // namespace
@@ -1458,16 +1466,14 @@ hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams&
                           int variant, const double* d_dt, double dx, double inv_dx, hipStream_t stream) {
     if (lv.noct == 0) return hipSuccess;
     if (precision != 0 && precision != 1) return hipErrorNotSupported;
+    if (variant < 0 || variant >= KERNEL_VARIANTS) return hipErrorInvalidValue;
     const bool mixed = precision == 1;
-    const bool oct = variant == 1;
     const bool exact = inv_dx != 0.0;
-    dim3 grid(oct ? (lv.noct + O_OCTS - 1) / O_OCTS : (lv.noct + G_OCTS - 1) / G_OCTS);
-    dim3 block(oct ? O_BLOCK : G_BLOCK);
     switch (riemann) {
-        case 0: launch_riemann<0>(oct, mixed, exact, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream); break;
-        case 1: launch_riemann<1>(oct, mixed, exact, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream); break;
-        case 2: launch_riemann<2>(oct, mixed, exact, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream); break;
-        case 4: launch_riemann<4>(oct, mixed, exact, grid, block, m, lv, gp, d_dt, dx, inv_dx, stream); break;
+        case 0: launch_riemann<0>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, stream); break;
+        case 1: launch_riemann<1>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, stream); break;
+        case 2: launch_riemann<2>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, stream); break;
+        case 4: launch_riemann<4>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, stream); break;
         default: return hipErrorNotSupported;
     }
     hipError_t e = hipGetLastError();

@@ -66,12 +66,14 @@ int courant_blocks(const LevelDev& lv);
 hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
                             double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream);
 // Godunov update of one level; dt is read from device memory (*d_dt).
-// variant 0: one thread per cell, octs strided across lanes; variant 1: the
-// eight cells of an oct in adjacent lanes, sharing by cross-lane shuffles.
+// variant 0: one thread per cell, octs strided across lanes; variants 1 and
+// 2: the eight cells of an oct in adjacent lanes, sharing by cross-lane
+// shuffles, 16 (variant 1) or 8 (variant 2) octs per workgroup.
 // dt is d_dt[0] for a single mesh and d_dt[box] for an ensemble.  inv_dx is
 // 1/dx when dx is a power of two (the FP64 form then multiplies by it, with
 // identical results), else 0; an ensemble reads dx and 1/dx per box and
 // passes inv_dx != 0 when every box has an exact reciprocal.
+constexpr int KERNEL_VARIANTS = 3;
 hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams& gp, int riemann, int precision,
                           int variant, const double* d_dt, double dx, double inv_dx, hipStream_t stream);
 // Ensemble Courant step: keys[b] <- min(keys[b], key(dt candidate)) over the

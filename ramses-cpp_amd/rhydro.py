@@ -22,6 +22,7 @@ RH_HOST_COHERENT, RH_DEVICE_RESIDENT = 0, 1
 RH_SYNC_TO_HOST, RH_SYNC_TO_DEVICE = 1, 2
 RH_UOLD, RH_UNEW = 1, 2
 TK_GODUNOV, TK_COURANT, TK_SET_UNEW, TK_SET_UOLD, TK_STEP_JOIN = range(5)
+RH_KERNEL_VARIANTS = 3  # 0 per-cell; 1, 2 oct-in-lanes with 16 / 8 octs per workgroup
 
 _IP = C.POINTER(C.c_int)
 _DP = C.POINTER(C.c_double)
@@ -530,7 +531,7 @@ class Solver:
 
     def autotune(self, ilevel, dt, dx, reps=3):
         best = C.c_int()
-        ms = np.zeros(2, np.float64)
+        ms = np.zeros(RH_KERNEL_VARIANTS, np.float64)
         self._chk(lib().rh_autotune(self._ctx, ilevel, dt, dx, reps, C.byref(best), _dp(ms)))
         return best.value, ms
 

@@ -32,7 +32,7 @@ def _boxes(seed, shapes=SHAPES, gamma=1.4):
     return out
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("riemann", ["llf", "hll", "hllc", "acoustic"])
 def test_ensemble_fp64_matches_oracle_per_box(riemann, variant):
     boxes = _boxes(100)

@@ -39,7 +39,7 @@ def _compare_steps(m0, solver_kw, level, nsteps, cf=0.8, mode=rhydro.RH_DEVICE_R
     s.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("riemann", RIEMANNS)
 @pytest.mark.parametrize("slope_type", range(9))
 def test_step_riemann_slope(riemann, slope_type, variant):
@@ -53,7 +53,7 @@ def test_step_interpol_type(interpol_type):
     _compare_steps(m, dict(gamma=1.4, slope_type=2, riemann="hllc", interpol_type=interpol_type), 4, 2)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("level", [2, 3, 5])
 def test_small_and_degenerate_levels(level, variant):
     # level 2 is a single oct whose every neighbour is itself (periodic).
@@ -80,7 +80,7 @@ def test_sedov_32_trajectory():
     _compare_steps(m, dict(gamma=1.4, slope_type=1, riemann="hllc"), level, 10)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_sedov_64_run_uniform_20_steps(variant):
     """Config 1 at its stated size (64^3, reference level 7), 20 steps
     through rh_run_uniform, against the oracle step by step."""
@@ -188,7 +188,8 @@ def test_autotune_keeps_state_and_results():
     s.bind(m)
     print("case autotune", flush=True)
     best, ms = s.autotune(level, 1e-3, 1 / 32, reps=2)
-    assert best in (0, 1) and s.variant() == best and (ms > 0).all()
+    assert best in range(rhydro.RH_KERNEL_VARIANTS) and s.variant() == best
+    assert ms.shape == (rhydro.RH_KERNEL_VARIANTS,) and (ms > 0).all()
     s.sync_to_host()
     assert_same(m.unew, mo.unew, "unew restored")
     om = oracle.OracleMesh(mo)
