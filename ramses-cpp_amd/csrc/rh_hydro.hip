@@ -710,6 +710,19 @@ template <typename T> RH_DEV void sel5(bool c, const T a[5], const T b[5], T out
     for (int i = 0; i < 5; ++i) out[i] = c ? a[i] : b[i];
 }
 
+// Workgroup index -> oct range, XCD-aware.  The dispatcher deals
+// workgroups round-robin to the 8 XCDs, each with its own L2, so workgroup
+// x runs on XCD x % 8.  Renumbering gives each XCD one contiguous share of
+// the list (XCD k takes ranges k*q.. in order, q = n/8, the first n%8 XCDs
+// one more), so octs that are neighbours in the list's Morton order, whose
+// cells each other's line gathers read, share an L2 instead of every
+// neighbour pair fetching the same lines from HBM on two XCDs.  A bijection
+// of [0, n) for any n.
+RH_DEV int xcd_block(int x, int n) {
+    const int xcd = x & 7, idx = x >> 3, q = n >> 3, r = n & 7;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
 // Octs per workgroup: 16 (two waves, kernel variant 1) or 8 (one wave,
 // variant 2); rh_autotune times both.
 template <int OCTS> constexpr int oct_block() { return OCTS * 8; }
@@ -729,7 +742,7 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
     __shared__ long long s_k[6][NT];
     const int t = threadIdx.x;
     const int ic0 = t & 7;
-    const int praw = blockIdx.x * OCTS + (t >> 3);
+    const int praw = xcd_block(blockIdx.x, gridDim.x) * OCTS + (t >> 3);
     const int noct = lv.noct;
     // Lanes past the end of the list work on the last oct (valid memory) and
     // write nothing, so every lane of an oct group takes part in the shuffles.
