@@ -51,9 +51,30 @@ PROFILED_VALU = {
 }
 
 
+def reference_arithmetic(rate: float, riemann: str, slope: int):
+    """The reference algorithm's FP64 operation rate when the Godunov kernel
+    updates `rate` cells/s, against MI355X's FP64 peak (which counts an FMA as
+    two operations; with fp-contract off every operation is its own
+    instruction)."""
+    ops = REFERENCE_FP64_OPS_PER_CELL.get((riemann, slope))
+    if ops is None:
+        return None
+    tflops = rate * ops / 1e12
+    return {"fp64_ops_per_cell_update": ops, "tflops": tflops, "peak_tflops": FP64_PEAK_TFLOPS,
+            "frac": tflops / FP64_PEAK_TFLOPS, "counted_by": "oracle/flopcount.cpp"}
+
+
 def kernel_family(variant: int) -> str:
     return "cell" if variant == 0 else "oct"
 PROFILE_NOTE = "profiles/godunov_level9_256cubed_hllc_minmod.md (rocprofv3 counter runs of this script)"
+
+# FP64 operations (add/sub, mul, div, sqrt) of the reference's godunov_fine per
+# cell-update on this workload's state, counted exactly by running the
+# oracle with a counting floating-point type (oracle/flopcount.cpp,
+# tests/test_oracle_flops.py), keyed by (riemann, slope_type).
+REFERENCE_FP64_OPS_PER_CELL = {("hllc", 1): 2412.6, ("hllc", 2): 2421.8, ("hll", 1): 2502.7, ("hll", 2): 2512.6,
+                               ("llf", 1): 2412.7, ("llf", 2): 2422.6}
+FP64_PEAK_TFLOPS = 78.6
 
 
 def cpu_model() -> str:
@@ -535,6 +556,7 @@ def main() -> int:
             "autotune": fp64["autotune"],
             "valu": PROFILED_VALU.get((level, args.riemann, args.slope, kernel_family(fp64["autotune"]["chosen_variant"]), "fp64")),
             "counters_from": PROFILE_NOTE,
+            "reference_arithmetic": reference_arithmetic(nleaf / (g_ms * 1e-3), args.riemann, args.slope),
             "mixed": mixed_info,
             "variants": variants,
             "ensemble": ens_info,

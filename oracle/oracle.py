@@ -432,3 +432,41 @@ def dump_snapshot(arrays: dict, dims: dict, root: str, **info) -> None:
     u = np.ascontiguousarray(arrays["uold"], np.float64)
     if _snap_lib().or_dump_snapshot(C.byref(t), _dp(u), C.byref(snapshot_info(**info)), os.fsencode(root)) != 0:
         raise OSError(f"oracle snapshot into {root} failed")
+
+
+# ---- FLOP count of godunov_fine (flopcount.cpp) ------------------------------
+
+FLOP_KINDS = ("add", "mul", "div", "sqrt", "cmp", "pow")
+_flops = None
+
+
+def _flops_lib() -> C.CDLL:
+    global _flops
+    if _flops is None:
+        path = os.path.join(HERE, "liboracle_flops.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.or_flop_count.argtypes = [C.c_int] * 6 + [_IP] * 5 + [_DP, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                                                 C.c_double, C.c_double, _DP,
+                                                                 C.POINTER(C.c_ulonglong)]
+        L.or_flop_count.restype = C.c_int
+        _flops = L
+    return _flops
+
+
+def flop_count(m, ilevel: int, gamma: float, slope_type: int, riemann: int, interpol_type: int, dt: float,
+               dx: float):
+    """set_unew + godunov_fine of `ilevel` on a copy of mesh `m` through the
+    oracle with a counting floating-point type.  Returns (unew after the
+    call, {kind: count} for the godunov_fine call); `m` is not modified."""
+    unew = np.zeros_like(m.uold)
+    counts = (C.c_ulonglong * 6)()
+    arrs = [np.ascontiguousarray(getattr(m, k), np.int32) for k in ("father", "son", "nbor", "next", "headl")]
+    uold = np.ascontiguousarray(m.uold, np.float64)
+    st = _flops_lib().or_flop_count(m.ncoarse, m.ngridmax, m.nx, m.ny, m.nz, m.nlevelmax, *(_ip(a) for a in arrs),
+                                    _dp(uold), ilevel, gamma, slope_type, riemann, interpol_type, dt, dx,
+                                    _dp(unew), counts)
+    if st != 0:
+        raise RuntimeError(f"counting oracle status {st}")
+    return unew, dict(zip(FLOP_KINDS, (int(c) for c in counts)))
