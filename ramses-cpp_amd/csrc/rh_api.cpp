@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -511,10 +512,55 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
     if ((st = get_level(c, ilevel, &lv))) return st;
     if ((st = ensure_dt(c, nsteps > 0 ? nsteps : 1))) return st;
     const rh::MeshDev m = mesh_dev(c);
-    // Step k is courant -> set_unew -> godunov_fine -> set_uold.  Between two
-    // steps, set_uold of the first and courant + set_unew of the second run
-    // as one fused pass (k_step_join), with the same state and time step.
-    for (int k = 0; k < nsteps; ++k) {
+    // A level of leaf cells whose neighbours are all at the same level (a
+    // uniform periodic level) steps with uold and unew exchanged instead of
+    // copied: the Godunov launch of step k reads the state from one buffer,
+    // writes the new state into the other (set_unew's copy is the buffer it
+    // reads, set_uold's copy is the exchange) and computes the next step's
+    // Courant candidates from the values it writes.  At the end the level's
+    // cells of both buffers hold the final state, as after set_uold.
+    const bool fused = c->variant != 0 && lv->ntgt == 0 && lv->nrec == 0 && lv->nleaf == 8LL * lv->noct &&
+                       lv->noct > 0;
+    if (fused) {
+        const int nb = rh::godunov_blocks(*lv, c->variant);
+        if ((st = ensure_part(c, nb + 1))) return st;
+        rh::MeshDev mk = m;
+        for (int k = 0; k < nsteps; ++k) {
+            if (k == 0 && (st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt))) return st;
+            const bool next = k + 1 < nsteps;
+            rh::FusedCourant fc;
+            fc.pre_from_uold = true;
+            fc.part = next ? c->d_part : nullptr;
+            fc.courant_factor = courant_factor;
+            {
+                Timed tm(c, TK_GODUNOV);
+                hipError_t e = rh::launch_godunov(mk, *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision,
+                                                  c->variant, c->d_dt + k, dx, exact_inverse(dx), c->stream, fc);
+                if (e == hipErrorNotSupported)
+                    return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
+                if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
+                tm.done();
+            }
+            if (next) {
+                Timed tm(c, TK_COURANT);
+                RH_HIP(c, rh::launch_courant_final(c->d_part, nb, c->p.boxlen / 1e-10, c->d_dt + k + 1, c->stream));
+                tm.done();
+            }
+            std::swap(mk.uold, mk.unew);
+        }
+        if (nsteps > 0) {
+            // mk.uold holds the final state; copy the level's cells into the
+            // other buffer (set_unew's copy, uold -> unew).
+            Timed tm(c, TK_SET_UOLD);
+            RH_HIP(c, rh::launch_set_unew(mk, *lv, c->stream));
+            tm.done();
+        }
+    }
+    // Otherwise step k is courant -> set_unew -> godunov_fine -> set_uold.
+    // Between two steps, set_uold of the first and courant + set_unew of the
+    // second run as one fused pass (k_step_join), with the same state and
+    // time step.
+    for (int k = 0; k < (fused ? 0 : nsteps); ++k) {
         if (k == 0) {
             if ((st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt))) return st;
             Timed tm(c, TK_SET_UNEW);

@@ -194,6 +194,20 @@ RH_DEV void block_min(double cand, double* part) {
     }
 }
 
+// Block minimum over an NT-thread block (NT a multiple of 64).
+template <int NT> RH_DEV void block_min_n(double cand, double* part) {
+    __shared__ double s_min[NT / 64];
+    const int t = threadIdx.x;
+    cand = wave_min(cand);
+    if ((t & 63) == 0) s_min[t >> 6] = cand;
+    __syncthreads();
+    if (t == 0) {
+        double v = s_min[0];
+        for (int w = 1; w < NT / 64; ++w) v = min_skip_nan(v, s_min[w]);
+        part[blockIdx.x] = v;
+    }
+}
+
 // Per leaf cell, then a block minimum (one thread per oct).
 __global__ __launch_bounds__(SBLOCK) void k_courant_partial(MeshDev m, LevelDev lv, double dx, double gamma,
                                                             double courant_factor, double* part) {
@@ -439,7 +453,7 @@ constexpr long long KMAX = 0x7fffffffffffffffLL;
 template <int NT>
 RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const double fs[5], double dtdx,
                           const GasParams& gp, double (*s_r)[NT], long long (*s_k)[NT], int t,
-                          const double* unew_pre = nullptr) {
+                          const double* unew_pre = nullptr, double* u_out = nullptr) {
     const long long KOWN = own_key * 8 + 6;
     long long ks[6];
     int ix[6];
@@ -481,6 +495,7 @@ RH_DEV void ordered_apply(const MeshDev& m, int cC, long long own_key, const dou
     const size_t ci = (size_t)cC - 1;
 #pragma unroll
     for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + ci] = u[iv];
+    if (u_out) copy5(u, u_out);
 }
 
 // One thread per cell of the level's octs; non-leaf cells exit.  For each
@@ -732,10 +747,19 @@ template <int OCTS> constexpr int oct_block() { return OCTS * 8; }
 // contributions and their keys.  The registers above what the arithmetic
 // needs hold the next dimension's neighbour states, gathered ahead of their
 // use.
+//
+// Fused step (a level whose cells are all leaves and whose neighbours are
+// all at the same level, stepped with uold and unew exchanged between
+// steps): `pre` is uold, which is what set_unew would have copied into unew,
+// and, except on the last step, each leaf cell also computes the Courant
+// candidate of its new state -- the state set_uold would copy into uold --
+// with a block minimum into cpart[blockIdx.x].  Otherwise pre is unew and
+// cpart is null.
 template <int RIEMANN, bool MIXED, bool EXACT, int OCTS>
 __global__ __launch_bounds__(OCTS * 8) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
-                                                          double dx, double inv_dx) {
+                                                          double dx, double inv_dx, const double* pre,
+                                                          double* cpart, double courant_factor) {
     using TC = typename std::conditional<MIXED, float, double>::type;
     constexpr int NT = oct_block<OCTS>();
     __shared__ double s_r[30][NT];
@@ -839,9 +863,10 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
             nx = across(d + 1);
             gather(nx, uXn, uX2n);
         } else {
-            // Last dimension: the cell's own unew entry (no other thread
-            // writes it in this kernel) takes the freed registers.
-            load_u(m.unew, m.ncell, cC, uXn);
+            // Last dimension: the cell's own unew entry before this update
+            // (no other thread writes it in this kernel) takes the freed
+            // registers.
+            load_u(pre, m.ncell, cC, uXn);
         }
         const int E = cur.E;
         const int4 rc = cur.rc;
@@ -961,8 +986,18 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
         s_k[slotX][t] = Kin;
     }
-    if (!leaf) return;
-    ordered_apply<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn);
+    if (!cpart) {
+        if (!leaf) return;
+        ordered_apply<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn);
+        return;
+    }
+    double cand = __builtin_inf();
+    if (leaf) {
+        double un[5];
+        ordered_apply<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn, un);
+        cand = courant_cand_u(m, un, cC, dx, gp.gamma, courant_factor);
+    }
+    block_min_n<NT>(cand, cpart);
 }
 
 // Reflux into coarser leaf cells, contributions in processing order.  Two
@@ -1411,6 +1446,15 @@ hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, dou
     return hipGetLastError();
 }
 
+int godunov_blocks(const LevelDev& lv, int variant) {
+    return variant == 1 ? (lv.noct + 15) / 16 : variant == 2 ? (lv.noct + 7) / 8 : (lv.noct + G_OCTS - 1) / G_OCTS;
+}
+
+hipError_t launch_courant_final(const double* d_part, int nparts, double dt_init, double* d_dt, hipStream_t stream) {
+    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(1024), 0, stream, d_part, nparts, dt_init, d_dt);
+    return hipGetLastError();
+}
+
 hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
                           double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream) {
     int nb = stream_blocks(lv.noct);
@@ -1451,13 +1495,14 @@ namespace {
 
 template <int R, bool MIXED, bool EXACT>
 void launch_form(int variant, const MeshDev& m, const LevelDev& lv, const GasParams& gp, const double* d_dt,
-                 double dx, double inv_dx, hipStream_t stream) {
+                 double dx, double inv_dx, const FusedCourant& fc, hipStream_t stream) {
+    const double* pre = fc.pre_from_uold ? m.uold : m.unew;
     if (variant == 1)
         hipLaunchKernelGGL((k_godunov_oct<R, MIXED, EXACT, 16>), dim3((lv.noct + 15) / 16), dim3(oct_block<16>()), 0,
-                           stream, m, lv, gp, d_dt, dx, inv_dx);
+                           stream, m, lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.courant_factor);
     else if (variant == 2)
         hipLaunchKernelGGL((k_godunov_oct<R, MIXED, EXACT, 8>), dim3((lv.noct + 7) / 8), dim3(oct_block<8>()), 0,
-                           stream, m, lv, gp, d_dt, dx, inv_dx);
+                           stream, m, lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.courant_factor);
     else
         hipLaunchKernelGGL((k_godunov<R, MIXED, EXACT>), dim3((lv.noct + G_OCTS - 1) / G_OCTS), dim3(G_BLOCK), 0,
                            stream, m, lv, gp, d_dt, dx, inv_dx);
@@ -1467,26 +1512,32 @@ void launch_form(int variant, const MeshDev& m, const LevelDev& lv, const GasPar
 // only the FP64 form has an exact-reciprocal instantiation.
 template <int R>
 void launch_riemann(int variant, bool mixed, bool exact, const MeshDev& m, const LevelDev& lv, const GasParams& gp,
-                    const double* d_dt, double dx, double inv_dx, hipStream_t stream) {
-    if (mixed) launch_form<R, true, false>(variant, m, lv, gp, d_dt, dx, inv_dx, stream);
-    else if (exact) launch_form<R, false, true>(variant, m, lv, gp, d_dt, dx, inv_dx, stream);
-    else launch_form<R, false, false>(variant, m, lv, gp, d_dt, dx, inv_dx, stream);
+                    const double* d_dt, double dx, double inv_dx, const FusedCourant& fc, hipStream_t stream) {
+    if (mixed) launch_form<R, true, false>(variant, m, lv, gp, d_dt, dx, inv_dx, fc, stream);
+    else if (exact) launch_form<R, false, true>(variant, m, lv, gp, d_dt, dx, inv_dx, fc, stream);
+    else launch_form<R, false, false>(variant, m, lv, gp, d_dt, dx, inv_dx, fc, stream);
 }
 
 }  // namespace
 
 hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams& gp, int riemann, int precision,
-                          int variant, const double* d_dt, double dx, double inv_dx, hipStream_t stream) {
+                          int variant, const double* d_dt, double dx, double inv_dx, hipStream_t stream,
+                          const FusedCourant& fc) {
     if (lv.noct == 0) return hipSuccess;
     if (precision != 0 && precision != 1) return hipErrorNotSupported;
     if (variant < 0 || variant >= KERNEL_VARIANTS) return hipErrorInvalidValue;
+    // The fused step needs the oct-in-lanes kernel, a single mesh, and a level
+    // with no coarser neighbours (no reflux targets, no records).
+    if ((fc.part || fc.pre_from_uold) &&
+        (variant == 0 || lv.obox || lv.ntgt != 0 || lv.nrec != 0 || lv.nleaf != 8LL * lv.noct))
+        return hipErrorInvalidValue;
     const bool mixed = precision == 1;
     const bool exact = inv_dx != 0.0;
     switch (riemann) {
-        case 0: launch_riemann<0>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, stream); break;
-        case 1: launch_riemann<1>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, stream); break;
-        case 2: launch_riemann<2>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, stream); break;
-        case 4: launch_riemann<4>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, stream); break;
+        case 0: launch_riemann<0>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, fc, stream); break;
+        case 1: launch_riemann<1>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, fc, stream); break;
+        case 2: launch_riemann<2>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, fc, stream); break;
+        case 4: launch_riemann<4>(variant, mixed, exact, m, lv, gp, d_dt, dx, inv_dx, fc, stream); break;
         default: return hipErrorNotSupported;
     }
     hipError_t e = hipGetLastError();

@@ -74,8 +74,24 @@ hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, dou
 // identical results), else 0; an ensemble reads dx and 1/dx per box and
 // passes inv_dx != 0 when every box has an exact reciprocal.
 constexpr int KERNEL_VARIANTS = 3;
+// Fused step of a single-level run: with pre_from_uold the launch reads the
+// cells' pre-update unew from uold (what set_unew copies into unew), and
+// with part non-null each leaf cell also adds the Courant candidate of its
+// new state (what set_uold copies into uold) to a block minimum in
+// part[godunov_blocks(lv, variant)].  Only for variants 1 and 2 on a single
+// mesh whose level has only leaf cells and same-level neighbours (no
+// records, no reflux targets).
+struct FusedCourant {
+    bool pre_from_uold = false;
+    double* part = nullptr;
+    double courant_factor = 0.0;
+};
 hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams& gp, int riemann, int precision,
-                          int variant, const double* d_dt, double dx, double inv_dx, hipStream_t stream);
+                          int variant, const double* d_dt, double dx, double inv_dx, hipStream_t stream,
+                          const FusedCourant& fc = FusedCourant{});
+int godunov_blocks(const LevelDev& lv, int variant);
+// Minimum of part[0..nparts) and dt_init into *d_dt (the Courant step).
+hipError_t launch_courant_final(const double* d_part, int nparts, double dt_init, double* d_dt, hipStream_t stream);
 // Ensemble Courant step: keys[b] <- min(keys[b], key(dt candidate)) over the
 // leaf cells of box b (keys ordered like the doubles they encode, NaN last).
 // join: fused with set_uold before it and set_unew after it, as

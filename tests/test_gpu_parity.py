@@ -257,3 +257,29 @@ def test_amr_run_reports_exhausted_grids():
     with pytest.raises(rhydro.RhError):
         rhydro.AmrRun(levelmin=4, levelmax=6, ngridmax=80, err_grad_d=0.1, err_grad_p=0.1,
                       regions=rhydro.sedov_regions(4))
+
+
+@pytest.mark.parametrize("precision", [rhydro.RH_PREC_FP64, rhydro.RH_PREC_MIXED])
+def test_run_uniform_exchanged_buffers_match_copies(precision):
+    """On a uniform periodic level rh_run_uniform with the oct-in-lanes
+    kernels exchanges uold and unew between steps and takes the next Courant
+    step from the Godunov launch; the per-cell kernel runs the copying
+    sequence (set_unew, godunov_fine, fused set_uold/Courant/set_unew).  Both
+    leave the same uold, unew and time steps after 1, 2 and 3 steps."""
+    level = 4
+    m0 = _random_mesh(level, seed=77)
+    for n in (1, 2, 3):
+        out = []
+        for variant in (0, 1, 2):
+            m = clone(m0)
+            s = rhydro.Solver(gamma=5.0 / 3.0, slope_type=1, riemann="hllc", precision=precision, variant=variant)
+            s.bind(m)
+            print(f"case exchanged buffers precision {precision} steps {n} variant {variant}", flush=True)
+            dts = s.run_uniform(level, 1 / 16, 0.8, n)
+            s.sync_to_host()
+            s.close()
+            out.append((np.array(dts), m.uold.copy(), m.unew.copy()))
+        for v in (1, 2):
+            assert_same(out[v][0], out[0][0], f"dt, {n} steps, variant {v}")
+            assert_same(out[v][1], out[0][1], f"uold, {n} steps, variant {v}")
+            assert_same(out[v][2], out[0][2], f"unew, {n} steps, variant {v}")
