@@ -33,12 +33,14 @@ BYTES_PER_CELL_UPDATE = 80  # read uold (5 x FP64) + write unew (5 x FP64)
 # what it reads for k_set_unew, a copy of known size, in the same run); keyed
 # by (level, riemann, slope_type, kernel, form), kernel "oct" for the
 # oct-in-lanes variants (profiled at 8 octs per workgroup, the variant the
-# autotuner keeps here; the 16-oct variant reads the same counts) and "cell"
-# for the per-cell one.  Counters cannot
+# autotuner keeps here, on the launches of the timed steps, which read the
+# pre-update state from uold and end with the next step's Courant
+# candidates; the 16-oct variant reads the same counts) and "cell" for the
+# per-cell one.  Counters cannot
 # be read while timing, so only profiled workloads report a figure.
 PROFILED_TRAFFIC = {
-    (9, "hllc", 1, "oct", "fp64"): 2.137e9,
-    (9, "hllc", 1, "oct", "mixed"): 2.141e9,
+    (9, "hllc", 1, "oct", "fp64"): 1.662e9,
+    (9, "hllc", 1, "oct", "mixed"): 1.662e9,
 }
 
 # VALU utilisation of the same launches from an SQ/GRBM counter run
@@ -46,8 +48,8 @@ PROFILED_TRAFFIC = {
 # instructions per wave (= per cell: 64 cells per wave) from SQ_INSTS_VALU /
 # SQ_WAVES, same key.
 PROFILED_VALU = {
-    (9, "hllc", 1, "oct", "fp64"): {"busy": 0.74, "insts_per_cell": 5737},
-    (9, "hllc", 1, "oct", "mixed"): {"busy": 0.69, "insts_per_cell": 4003},
+    (9, "hllc", 1, "oct", "fp64"): {"busy": 0.76, "insts_per_cell": 5980},
+    (9, "hllc", 1, "oct", "mixed"): {"busy": 0.72, "insts_per_cell": 4230},
 }
 
 
