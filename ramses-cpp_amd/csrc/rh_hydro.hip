@@ -753,13 +753,15 @@ template <int OCTS> constexpr int oct_block() { return OCTS * 8; }
 // steps): `pre` is uold, which is what set_unew would have copied into unew,
 // and, except on the last step, each leaf cell also computes the Courant
 // candidate of its new state -- the state set_uold would copy into uold --
-// with a block minimum into cpart[blockIdx.x].  Otherwise pre is unew and
-// cpart is null.
+// with a block minimum into cpart[blockIdx.x] (a single mesh) or, for an
+// ensemble, into its box's key in ckeys as k_courant_boxes does.  Otherwise
+// pre is unew and cpart and ckeys are null.
 template <int RIEMANN, bool MIXED, bool EXACT, int OCTS>
 __global__ __launch_bounds__(OCTS * 8) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                                                           double dx, double inv_dx, const double* pre,
-                                                          double* cpart, double courant_factor) {
+                                                          double* cpart, unsigned long long* ckeys,
+                                                          double courant_factor) {
     using TC = typename std::conditional<MIXED, float, double>::type;
     constexpr int NT = oct_block<OCTS>();
     __shared__ double s_r[30][NT];
@@ -986,7 +988,7 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
         s_k[slotX][t] = Kin;
     }
-    if (!cpart) {
+    if (!cpart && !ckeys) {
         if (!leaf) return;
         ordered_apply<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn);
         return;
@@ -997,7 +999,27 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         ordered_apply<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn, un);
         cand = courant_cand_u(m, un, cC, dx, gp.gamma, courant_factor);
     }
-    block_min_n<NT>(cand, cpart);
+    if (cpart) {
+        block_min_n<NT>(cand, cpart);
+        return;
+    }
+    // Ensemble: one atomic per workgroup when its octs are all in one box,
+    // else one per cell (the minimum of a set is order-independent).
+    const int pfirst = praw - (t >> 3);
+    const int b0 = lv.obox[pfirst], b1 = lv.obox[min(pfirst + OCTS, noct) - 1];
+    if (b0 == b1) {
+        __shared__ double s_bmin[NT / 64];
+        cand = wave_min(cand);
+        if ((t & 63) == 0) s_bmin[t >> 6] = cand;
+        __syncthreads();
+        if (t == 0) {
+            double v = s_bmin[0];
+            for (int w = 1; w < NT / 64; ++w) v = min_skip_nan(v, s_bmin[w]);
+            atomicMin(&ckeys[b0], dt_key(v));
+        }
+    } else if (cand != __builtin_inf()) {
+        atomicMin(&ckeys[lv.obox[p]], dt_key(cand));
+    }
 }
 
 // Reflux into coarser leaf cells, contributions in processing order.  Two
@@ -1499,10 +1521,10 @@ void launch_form(int variant, const MeshDev& m, const LevelDev& lv, const GasPar
     const double* pre = fc.pre_from_uold ? m.uold : m.unew;
     if (variant == 1)
         hipLaunchKernelGGL((k_godunov_oct<R, MIXED, EXACT, 16>), dim3((lv.noct + 15) / 16), dim3(oct_block<16>()), 0,
-                           stream, m, lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.courant_factor);
+                           stream, m, lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.keys, fc.courant_factor);
     else if (variant == 2)
         hipLaunchKernelGGL((k_godunov_oct<R, MIXED, EXACT, 8>), dim3((lv.noct + 7) / 8), dim3(oct_block<8>()), 0,
-                           stream, m, lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.courant_factor);
+                           stream, m, lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.keys, fc.courant_factor);
     else
         hipLaunchKernelGGL((k_godunov<R, MIXED, EXACT>), dim3((lv.noct + G_OCTS - 1) / G_OCTS), dim3(G_BLOCK), 0,
                            stream, m, lv, gp, d_dt, dx, inv_dx);
@@ -1528,9 +1550,10 @@ hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams&
     if (variant < 0 || variant >= KERNEL_VARIANTS) return hipErrorInvalidValue;
     // The fused step needs the oct-in-lanes kernel, a single mesh, and a level
     // with no coarser neighbours (no reflux targets, no records).
-    if ((fc.part || fc.pre_from_uold) &&
-        (variant == 0 || lv.obox || lv.ntgt != 0 || lv.nrec != 0 || lv.nleaf != 8LL * lv.noct))
+    if ((fc.part || fc.keys || fc.pre_from_uold) &&
+        (variant == 0 || lv.ntgt != 0 || lv.nrec != 0 || lv.nleaf != 8LL * lv.noct))
         return hipErrorInvalidValue;
+    if ((fc.part && (lv.obox || fc.keys)) || (fc.keys && !lv.obox)) return hipErrorInvalidValue;
     const bool mixed = precision == 1;
     const bool exact = inv_dx != 0.0;
     switch (riemann) {

@@ -74,16 +74,18 @@ hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, dou
 // identical results), else 0; an ensemble reads dx and 1/dx per box and
 // passes inv_dx != 0 when every box has an exact reciprocal.
 constexpr int KERNEL_VARIANTS = 3;
-// Fused step of a single-level run: with pre_from_uold the launch reads the
-// cells' pre-update unew from uold (what set_unew copies into unew), and
-// with part non-null each leaf cell also adds the Courant candidate of its
-// new state (what set_uold copies into uold) to a block minimum in
-// part[godunov_blocks(lv, variant)].  Only for variants 1 and 2 on a single
-// mesh whose level has only leaf cells and same-level neighbours (no
-// records, no reflux targets).
+// Fused step of a single-level run or an ensemble: with pre_from_uold the
+// launch reads the cells' pre-update unew from uold (what set_unew copies
+// into unew), and each leaf cell also takes the Courant candidate of its new
+// state (what set_uold copies into uold) into a block minimum in
+// part[godunov_blocks(lv, variant)] (single mesh, part non-null) or into
+// its box's key in keys (ensemble, keys non-null; as launch_courant_boxes).
+// Only for variants 1 and 2 on levels with only leaf cells and same-level
+// neighbours (no records, no reflux targets).
 struct FusedCourant {
     bool pre_from_uold = false;
     double* part = nullptr;
+    unsigned long long* keys = nullptr;
     double courant_factor = 0.0;
 };
 hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams& gp, int riemann, int precision,
