@@ -74,9 +74,9 @@ int rh_timers_read(rh_ctx* ctx, int kernel, double* ms_total, long long* count);
 int rh_timers_reset(rh_ctx* ctx);
 
 /* Godunov kernel variant of a context: 0 = one thread per cell with the
- * octs of a workgroup strided across lanes; 1 (default) = the eight cells of
- * an oct in adjacent lanes, sharing sibling states and fluxes by cross-lane
- * shuffles, 16 octs per workgroup; 2 = the same with 8 octs per workgroup.
+ * octs of a workgroup strided across lanes; 1 = the eight cells of an oct in
+ * adjacent lanes, sharing sibling states and fluxes by cross-lane shuffles,
+ * 16 octs per workgroup; 2 (default) = the same with 8 octs per workgroup.
  * All produce identical results. */
 #define RH_KERNEL_VARIANTS 3
 int rh_set_kernel_variant(rh_ctx* ctx, int variant);

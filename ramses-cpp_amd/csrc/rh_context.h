@@ -55,7 +55,7 @@ struct rh_ctx {
     double* d_u1 = nullptr;
     double* d_u2 = nullptr;
     int interpol_cap = 0;
-    int variant = 1;  // Godunov kernel variant (rh_set_kernel_variant)
+    int variant = 2;  // Godunov kernel variant (rh_set_kernel_variant)
     bool timers = false;
     double t_ms[rhi::TK_N] = {0, 0, 0, 0, 0};
     double table_ms = 0.0;  // host time spent rebuilding level tables
