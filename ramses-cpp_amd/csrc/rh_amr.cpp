@@ -898,9 +898,10 @@ int amr_step(rh_amr* a, int il, int icount) {
     const hipStream_t stream = a->ctx->stream;
     {
         Phase ph{a, PH_HYDRO};
-        RH_ACHK(courant_dev(a->ctx, il, dx, a->gamma, a->ap.courant_factor, a->d_times + rh::AMR_T_DTC));
+        // The Courant pass also does the level's set_unew (it copies the
+        // uold values the Courant step reads).
+        RH_ACHK(courant_dev(a->ctx, il, dx, a->gamma, a->ap.courant_factor, a->d_times + rh::AMR_T_DTC, true));
         RH_AHIP(a, rh::launch_amr_times(a->d_times, 0, il, a->levelmin, (double)a->nsubcycle[il - 1], 0.0, stream));
-        RH_ACHK(rh_set_unew(a->ctx, il));
     }
     const int nsub = il < (int)a->nsubcycle.size() ? a->nsubcycle[il] : 1;
     if (il < a->nlevelmax) {

@@ -208,7 +208,11 @@ template <int NT> RH_DEV void block_min_n(double cand, double* part) {
     }
 }
 
-// Per leaf cell, then a block minimum (one thread per oct).
+// Per leaf cell, then a block minimum (one thread per oct).  SET_UNEW: the
+// same pass also copies every cell of the oct from uold into unew
+// (set_unew, which reads the same values), for a driver that runs the two
+// back to back.
+template <bool SET_UNEW>
 __global__ __launch_bounds__(SBLOCK) void k_courant_partial(MeshDev m, LevelDev lv, double dx, double gamma,
                                                             double courant_factor, double* part) {
     const int p = blockIdx.x * SBLOCK + threadIdx.x;
@@ -216,8 +220,19 @@ __global__ __launch_bounds__(SBLOCK) void k_courant_partial(MeshDev m, LevelDev 
     if (p < lv.noct) {
         const int g = lv.grid[p];
         const int lm = lv.leaf[p];
-        for (int ic0 = 0; ic0 < 8; ++ic0)
-            if ((lm >> ic0) & 1) cand = min_skip_nan(cand, courant_cand(m, cell_id(m, g, ic0), dx, gamma, courant_factor));
+        for (int ic0 = 0; ic0 < 8; ++ic0) {
+            const int id = cell_id(m, g, ic0);
+            if constexpr (SET_UNEW) {
+                double u[5];
+                load_u(m.uold, m.ncell, id, u);
+                const size_t c = (size_t)id - 1;
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
+                if ((lm >> ic0) & 1) cand = min_skip_nan(cand, courant_cand_u(m, u, id, dx, gamma, courant_factor));
+            } else if ((lm >> ic0) & 1) {
+                cand = min_skip_nan(cand, courant_cand(m, id, dx, gamma, courant_factor));
+            }
+        }
     }
     block_min(cand, part);
 }
@@ -1478,12 +1493,16 @@ hipError_t launch_courant_final(const double* d_part, int nparts, double dt_init
 }
 
 hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
-                          double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream) {
+                          double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream,
+                          bool set_unew) {
     int nb = stream_blocks(lv.noct);
     if (nb > npart) return hipErrorInvalidValue;
-    if (nb > 0)
-        hipLaunchKernelGGL(k_courant_partial, dim3(nb), dim3(SBLOCK), 0, stream, m, lv, dx, gamma, courant_factor,
-                           d_part);
+    if (nb > 0 && set_unew)
+        hipLaunchKernelGGL(k_courant_partial<true>, dim3(nb), dim3(SBLOCK), 0, stream, m, lv, dx, gamma,
+                           courant_factor, d_part);
+    else if (nb > 0)
+        hipLaunchKernelGGL(k_courant_partial<false>, dim3(nb), dim3(SBLOCK), 0, stream, m, lv, dx, gamma,
+                           courant_factor, d_part);
     hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(1024), 0, stream, (const double*)d_part, nb, dt_init, d_dt);
     return hipGetLastError();
 }
