@@ -899,9 +899,14 @@ int amr_step(rh_amr* a, int il, int icount) {
     {
         Phase ph{a, PH_HYDRO};
         // The Courant pass also does the level's set_unew (it copies the
-        // uold values the Courant step reads).
-        RH_ACHK(courant_dev(a->ctx, il, dx, a->gamma, a->ap.courant_factor, a->d_times + rh::AMR_T_DTC, true));
-        RH_AHIP(a, rh::launch_amr_times(a->d_times, 0, il, a->levelmin, (double)a->nsubcycle[il - 1], 0.0, stream));
+        // uold values the Courant step reads), and its reduction the time
+        // step bookkeeping (op 0 of k_amr_times).
+        rh::AmrTimesArgs at;
+        at.T = a->d_times;
+        at.il = il;
+        at.lmin = a->levelmin;
+        at.nsub_prev = (double)a->nsubcycle[il - 1];
+        RH_ACHK(courant_dev(a->ctx, il, dx, a->gamma, a->ap.courant_factor, a->d_times + rh::AMR_T_DTC, true, at));
     }
     const int nsub = il < (int)a->nsubcycle.size() ? a->nsubcycle[il] : 1;
     if (il < a->nlevelmax) {

@@ -242,13 +242,13 @@ void unbind(rh_ctx* c) {
 }
 
 int do_courant(rh_ctx* c, const rh::LevelDev* lv, double dx, double gamma, double cf, double* d_out,
-               bool set_unew = false) {
+               bool set_unew = false, const rh::AmrTimesArgs& at = rh::AmrTimesArgs{}) {
     int st = ensure_part(c, rh::courant_blocks(*lv) + 1);
     if (st) return st;
     double dt_init = c->p.boxlen / 1e-10;
     Timed tm(c, TK_COURANT);
     RH_HIP(c, rh::launch_courant(mesh_dev(c), *lv, dx, gamma, cf, dt_init, c->d_part, c->npart, d_out, c->stream,
-                                 set_unew));
+                                 set_unew, at));
     tm.done();
     return RH_OK;
 }
@@ -268,14 +268,14 @@ int do_godunov(rh_ctx* c, const rh::LevelDev* lv, const double* d_dt, double dx)
 // rh_godunov_fine / rh_courant, without the host round trip.  Single mesh in
 // device-resident mode only.
 int courant_dev(rh_ctx* c, int ilevel, double dx, double gamma, double courant_factor, double* d_out,
-                bool set_unew) {
+                bool set_unew, const rh::AmrTimesArgs& at) {
     int st = check_bound(c);
     if (st) return st;
     if (c->mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "device time steps need a device-resident mesh");
     if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
     const rh::LevelDev* lv;
     if ((st = get_level(c, ilevel, &lv))) return st;
-    return do_courant(c, lv, dx, gamma, courant_factor, d_out, set_unew);
+    return do_courant(c, lv, dx, gamma, courant_factor, d_out, set_unew, at);
 }
 
 int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx) {

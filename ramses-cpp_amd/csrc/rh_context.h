@@ -98,10 +98,11 @@ int check_bound(rh_ctx* c);
 // Host-coherent mode: uploads the host state (and f) before a call.
 int pre_call(rh_ctx* c);
 // Courant minimum of a level into d_out (with set_unew: the level's
-// set_unew in the same pass), and godunov_fine with dt read from d_dt, both
+// set_unew in the same pass; with at.T: the adaptive run's time-step
+// bookkeeping after it), and godunov_fine with dt read from d_dt, both
 // device pointers (device-resident single mesh; no host wait).
 int courant_dev(rh_ctx* c, int ilevel, double dx, double gamma, double courant_factor, double* d_out,
-                bool set_unew = false);
+                bool set_unew = false, const rh::AmrTimesArgs& at = rh::AmrTimesArgs{});
 int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx);
 // Frees every device buffer of the current binding (single or ensemble).
 void unbind(rh_ctx* c);

@@ -355,7 +355,12 @@ __global__ __launch_bounds__(256) void k_keys_to_dt(const unsigned long long* ke
 // One workgroup of 1024 threads; each keeps four independent minima so that
 // four loads are in flight per round instead of a chain of dependent ones
 // (the partial minima number one per 32 octs, 65,536 at 2^21 octs).
-__global__ __launch_bounds__(1024) void k_courant_final(const double* part, int npart, double dt_init, double* dt) {
+RH_DEV void amr_times(double* T, int op, int il, int lmin, double nsub_prev, double nsub);
+
+// at.T non-null: after storing the minimum, the adaptive run's time-step
+// bookkeeping of the level step (op 0 of k_amr_times) in the same thread.
+__global__ __launch_bounds__(1024) void k_courant_final(const double* part, int npart, double dt_init, double* dt,
+                                                        AmrTimesArgs at) {
     __shared__ double s_min[16];
     const int t = threadIdx.x;
     double v0 = __builtin_inf(), v1 = v0, v2 = v0, v3 = v0;
@@ -375,6 +380,7 @@ __global__ __launch_bounds__(1024) void k_courant_final(const double* part, int 
         double r = dt_init;
         for (int w = 0; w < 16; ++w) r = min_skip_nan(r, s_min[w]);
         *dt = r;
+        if (at.T) amr_times(at.T, 0, at.il, at.lmin, at.nsub_prev, 0.0);
     }
 }
 
@@ -1276,9 +1282,7 @@ __global__ __launch_bounds__(64) void k_new_octs(MeshDev m, int n, const int* __
 //   il > lmin, else dtc (std::min's operand order); used[il] = dtnew[il].
 // op 1: a level without finer octs: dtold/dtnew[il+1] = dtnew[il]/nsub, then
 //   t += used[il].  op 2: the finest level: t += used[il].
-__global__ __launch_bounds__(64) void k_amr_times(double* T, int op, int il, int lmin, double nsub_prev,
-                                                  double nsub) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+RH_DEV void amr_times(double* T, int op, int il, int lmin, double nsub_prev, double nsub) {
     double* dtnew = T + AMR_T_DTNEW;
     double* dtold = T + AMR_T_DTOLD;
     double* used = T + AMR_T_USED;
@@ -1299,6 +1303,12 @@ __global__ __launch_bounds__(64) void k_amr_times(double* T, int op, int il, int
         }
         T[AMR_T_TIME] = T[AMR_T_TIME] + used[il];
     }
+}
+
+__global__ __launch_bounds__(64) void k_amr_times(double* T, int op, int il, int lmin, double nsub_prev,
+                                                  double nsub) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    amr_times(T, op, il, lmin, nsub_prev, nsub);
 }
 
 // restrict_fine (TreeUpdater.cpp:195-215): the father cell of every oct of
@@ -1479,7 +1489,8 @@ hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, dou
     if (nb > npart) return hipErrorInvalidValue;
     if (nb > 0)
         hipLaunchKernelGGL(k_step_join, dim3(nb), dim3(SBLOCK), 0, stream, m, lv, dx, gamma, courant_factor, d_part);
-    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(1024), 0, stream, (const double*)d_part, nb, dt_init, d_dt);
+    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(1024), 0, stream, (const double*)d_part, nb, dt_init, d_dt,
+                       AmrTimesArgs{});
     return hipGetLastError();
 }
 
@@ -1488,13 +1499,15 @@ int godunov_blocks(const LevelDev& lv, int variant) {
 }
 
 hipError_t launch_courant_final(const double* d_part, int nparts, double dt_init, double* d_dt, hipStream_t stream) {
-    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(1024), 0, stream, d_part, nparts, dt_init, d_dt);
+    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(1024), 0, stream, d_part, nparts, dt_init, d_dt,
+                       AmrTimesArgs{});
     return hipGetLastError();
 }
 
 hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
                           double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream,
-                          bool set_unew) {
+                          bool set_unew, const AmrTimesArgs& at) {
+    if (at.T && (at.il < 1 || at.il + 1 >= AMR_T_LEVELS)) return hipErrorInvalidValue;
     int nb = stream_blocks(lv.noct);
     if (nb > npart) return hipErrorInvalidValue;
     if (nb > 0 && set_unew)
@@ -1503,7 +1516,8 @@ hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, doubl
     else if (nb > 0)
         hipLaunchKernelGGL(k_courant_partial<false>, dim3(nb), dim3(SBLOCK), 0, stream, m, lv, dx, gamma,
                            courant_factor, d_part);
-    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(1024), 0, stream, (const double*)d_part, nb, dt_init, d_dt);
+    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(1024), 0, stream, (const double*)d_part, nb, dt_init, d_dt,
+                       at);
     return hipGetLastError();
 }
 

@@ -57,12 +57,20 @@ hipError_t launch_set_uold(const MeshDev& m, const LevelDev& lv, hipStream_t str
 // (synchro_hydro_fine).  m.f null means f = 0.
 hipError_t launch_gravity_source(const MeshDev& m, const LevelDev& lv, double dt, hipStream_t stream);
 hipError_t launch_synchro(const MeshDev& m, const LevelDev& lv, double dt, double gamma, hipStream_t stream);
+// An adaptive run's level step whose time-step bookkeeping (op 0 of
+// launch_amr_times on table T) the Courant reduction does once its minimum
+// is stored; T null: none.
+struct AmrTimesArgs {
+    double* T = nullptr;
+    int il = 0, lmin = 0;
+    double nsub_prev = 1.0;
+};
 // Courant minimum into *d_dt (device).  d_part has >= courant_blocks(lv) doubles.
 // set_unew: the same pass also does set_unew (uold -> unew on every cell of
 // the level's octs).
 hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
                           double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream,
-                          bool set_unew = false);
+                          bool set_unew = false, const AmrTimesArgs& at = AmrTimesArgs{});
 int courant_blocks(const LevelDev& lv);
 // set_uold of one step, then the Courant minimum (into *d_dt) and set_unew of
 // the next, fused into one pass over the level; same d_part requirement.
