@@ -139,15 +139,23 @@ void refresh_mesh(rh_amr* a) {
     m.mesh_epoch = a->epoch;
 }
 
-// The tree changed: the context's level tables are stale.  The device
-// copies of father/nbor stay current without an upload: kernels read them
-// only for listed grids, k_new_octs writes the entries of every grid it
-// creates, and a removed grid's stale entries are unused until a new oct
-// takes the grid (and writes them again).
-void tree_changed(rh_amr* a) {
+// Level list k gained or lost octs.  The tables of level il are built from
+// lists il-1 (the coarser neighbours of its records and their neighbours),
+// il (its octs and same-level neighbours) and il+1 (its leaf masks), so the
+// tables of levels k-1, k and k+1 are stale and the others stay valid: they
+// are carried to the new mesh epoch.  The device copies of father/nbor stay
+// current without an upload: kernels read them only for listed grids,
+// k_new_octs writes the entries of every grid it creates, and a removed
+// grid's stale entries are unused until a new oct takes the grid (and
+// writes them again).
+void tree_changed(rh_amr* a, int k) {
+    const long long old = a->epoch;
     a->epoch++;
     a->ctx->mesh.mesh_epoch = a->epoch;
     a->mesh.mesh_epoch = a->epoch;
+    std::vector<rhi::DevLevel>& levels = a->ctx->levels;
+    for (int il = 1; il < (int)levels.size(); ++il)
+        if (levels[il].epoch == old && (il < k - 1 || il > k + 1)) levels[il].epoch = a->epoch;
 }
 
 // The whole host father/nbor once, before the first kernel that reads them
@@ -750,7 +758,7 @@ int make_grid_fine(rh_amr* a, int il) {
     RH_AHIP(a, rh::launch_new_octs(mesh_dev(a->ctx), n, a->d_cells, gas(a->ctx, a->gamma), a->hp.interpol_var,
                                    a->d_father, a->d_nbor, a->ctx->stream));
     RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));  // idx goes out of scope
-    tree_changed(a);
+    tree_changed(a, il + 1);
     return RH_OK;
 }
 
@@ -783,7 +791,7 @@ void remove_grid_fine(rh_amr* a, int il) {
         }
         ig = nxt;
     }
-    if (changed) tree_changed(a);
+    if (changed) tree_changed(a, il);
 }
 
 // TreeUpdater.cpp:195-215 on the device.

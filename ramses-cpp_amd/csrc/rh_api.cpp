@@ -82,15 +82,16 @@ int get_level(rh_ctx* c, int ilevel, const rh::LevelDev** out) {
             c->table_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
     } clock{c};
-    if (!c->topo_valid || c->topo.epoch != c->mesh.mesh_epoch) {
-        std::string e;
-        int st = rh::build_topo_index(c->mesh, c->topo, e);
-        if (st != RH_OK) return fail(c, st, e);
-        c->topo_valid = true;
-    }
     if ((int)c->levels.size() <= ilevel) c->levels.resize(ilevel + 1);
     DevLevel& L = c->levels[ilevel];
     if (L.epoch != c->mesh.mesh_epoch) {
+        // The topology index is built only when some level's tables are.
+        if (!c->topo_valid || c->topo.epoch != c->mesh.mesh_epoch) {
+            std::string e;
+            int st = rh::build_topo_index(c->mesh, c->topo, e);
+            if (st != RH_OK) return fail(c, st, e);
+            c->topo_valid = true;
+        }
         free_level(L);
         rh::LevelHost h;
         std::string e;
