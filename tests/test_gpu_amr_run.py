@@ -47,6 +47,24 @@ def test_sedov_amr_run(riemann):
     o.close()
 
 
+def test_sedov_amr_run_four_levels():
+    """Four refined levels and six coarse steps: the blast grows, so level
+    lists gain and lose octs at every depth while the level tables of the
+    lists not next to a change are kept from one step to the next."""
+    kw = dict(levelmin=4, levelmax=7, ngridmax=20000, err_grad_d=0.1, err_grad_p=0.1, riemann="hllc",
+              regions=rhydro.sedov_regions(4))
+    print("case amr run sedov levels 4-7", flush=True)
+    g, o = _pair(**kw)
+    _compare(g.state(), o.state(), "initial mesh")
+    for k in range(6):
+        g.step(1)
+        assert o.step(1) == 0
+        _compare(g.state(), o.state(), f"coarse step {k + 1}")
+    assert g.state()["numbl"][7] > 0
+    g.close()
+    o.close()
+
+
 def test_kelvin_helmholtz_amr_run():
     kw = dict(levelmin=5, levelmax=6, ngridmax=4000, err_grad_d=0.2, slope_type=1, riemann="hllc", ic="kh",
               kh_amp=0.05)
