@@ -348,34 +348,45 @@ int walk_neighbour(const rh_amr* a, int g, int ic0, const int o[3], double w) {
     return cell(a, g, bits + 1);
 }
 
-// Every 27-neighbour of a cell exists at its level and is refined.
+// Every 27-neighbour of a cell exists at its level and is refined
+// (AmrGrid.cpp:474-511 finds them by coordinates).
 bool neighbours_refined_by_search(const rh_amr* a, int icell);
 
-bool neighbours_refined(const rh_amr* a, int icell) {
-    // The coordinate search scales positions by boxlen/max(nx,ny,nz) at the
-    // coarse level and by boxlen/nx below it; the two agree with the links
-    // only when the coarse grid is cubic.
-    if (icell <= a->ncoarse || a->nx != a->ny || a->nx != a->nz) return neighbours_refined_by_search(a, icell);
-    const int g = ((icell - a->ncoarse - 1) % a->ngridmax) + 1;
-    const int ic0 = (icell - a->ncoarse - 1) / a->ngridmax;
-    // Width of an oct of this cell's depth D (father chain) in coarse-cell
-    // units: a depth-1 oct fills one coarse cell, and each refinement halves
-    // it (make_grid_fine's offsets), so 1/2^(D-1).
-    const int level = depth_of(a, g);
-    const double w = 2.0 / (double)(1 << level);
+// The 4x4x4 cells around oct g at g's level, positions -1..2 per dimension
+// relative to g's cell 0 (index ((z+1)*4 + y+1)*4 + x+1), each reached
+// through the nbor/son links as walk_neighbour does, 0 where a walk does not
+// land exactly.  w is the width of an oct of g's depth D in coarse-cell
+// units: a depth-1 oct fills one coarse cell and each refinement halves it
+// (make_grid_fine's offsets), so 2/2^D.
+void cells_around(const rh_amr* a, int g, int out[64]) {
+    const double w = 2.0 / (double)(1 << depth_of(a, g));
+    for (int z = -1; z <= 2; ++z)
+        for (int y = -1; y <= 2; ++y)
+            for (int x = -1; x <= 2; ++x) {
+                const int o[3] = {x, y, z};
+                out[((z + 1) * 4 + (y + 1)) * 4 + (x + 1)] = walk_neighbour(a, g, 0, o, w);
+            }
+}
+
+// neighbours_refined_by_search for cell ic0 (0-based) of oct g, from g's
+// cells_around: a walk from cell ic0 with offset o takes the same links as
+// one from cell 0 with offset o + bits(ic0), so the block holds the 27
+// neighbours.  Any neighbour the links do not reach exactly (a coarser
+// region) sends the cell to the search.
+bool neighbours_refined_in_block(const rh_amr* a, int g, int ic0, const int block[64]) {
+    const int bx = ic0 & 1, by = (ic0 >> 1) & 1, bz = (ic0 >> 2) & 1;
     bool any_fallback = false;
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
-                const int o[3] = {dx, dy, dz};
-                const int c = walk_neighbour(a, g, ic0, o, w);
+                const int c = block[((bz + dz + 1) * 4 + (by + dy + 1)) * 4 + (bx + dx + 1)];
                 if (c == 0) {
                     any_fallback = true;
                     continue;
                 }
                 if (a->son[c - 1] == 0) return false;
             }
-    return any_fallback ? neighbours_refined_by_search(a, icell) : true;
+    return any_fallback ? neighbours_refined_by_search(a, a->ncoarse + ic0 * a->ngridmax + g) : true;
 }
 
 bool neighbours_refined_by_search(const rh_amr* a, int icell) {
@@ -580,12 +591,44 @@ void test_flag(rh_amr* a, int il, const std::vector<int>& grids) {
 }
 
 void ensure_ref_rules(rh_amr* a, const std::vector<int>& grids) {
-    parallel_for((int)grids.size(), [&](int k) {
-        const int ig = grids[k];
-        int fc = a->father[ig - 1];
-        bool ok = fc > 0 && neighbours_refined(a, fc);
-        if (!ok)
-            for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
+    // Octs of a list whose fathers are cells of one oct G were made together,
+    // so they sit in runs; a run takes G's 4x4x4 cells once and finds each
+    // father's 27 neighbours in it.  Coarse fathers, and a non-cubic coarse
+    // grid (where the coordinate search scales positions by boxlen/max(n) at
+    // the coarse level and by boxlen/nx below it), take the search.
+    const bool cubic = a->nx == a->ny && a->nx == a->nz;
+    auto father_oct = [&](int ig) {
+        const int fc = a->father[ig - 1];
+        return fc > a->ncoarse ? ((fc - a->ncoarse - 1) % a->ngridmax) + 1 : 0;
+    };
+    std::vector<int> start;
+    start.reserve(grids.size() / 4 + 2);
+    for (size_t k = 0; k < grids.size(); ++k)
+        if (k == 0 || father_oct(grids[k]) != father_oct(grids[k - 1]) || father_oct(grids[k]) == 0)
+            start.push_back((int)k);
+    start.push_back((int)grids.size());
+    parallel_for((int)start.size() - 1, [&](int r) {
+        int block[64];
+        bool have = false;
+        for (int k = start[r]; k < start[r + 1]; ++k) {
+            const int ig = grids[k];
+            const int fc = a->father[ig - 1];
+            bool ok;
+            if (fc <= 0) {
+                ok = false;
+            } else if (fc <= a->ncoarse || !cubic) {
+                ok = neighbours_refined_by_search(a, fc);
+            } else {
+                const int g = ((fc - a->ncoarse - 1) % a->ngridmax) + 1;
+                if (!have) {
+                    cells_around(a, g, block);
+                    have = true;
+                }
+                ok = neighbours_refined_in_block(a, g, (fc - a->ncoarse - 1) / a->ngridmax, block);
+            }
+            if (!ok)
+                for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
+        }
     });
 }
 
