@@ -69,8 +69,12 @@ FLOP_COMPARE(!=)
 }  // namespace flopcount
 
 // The oracle calls std::sqrt, std::fabs and std::pow on its floating-point
-// values; these overloads take the counting type.
-namespace std {
+// values.  While it is compiled here, its `std` names this namespace, which
+// adds overloads for the counting type and reaches everything else of std
+// through the using-directive (the standard headers the oracle includes are
+// included above, before the renaming).
+namespace flopstd {
+using namespace std;
 inline flopcount::Flop sqrt(flopcount::Flop a) {
     ++flopcount::g_tally.sqrt;
     return flopcount::Flop(std::sqrt(a.v));
@@ -84,12 +88,14 @@ inline flopcount::Flop pow(flopcount::Flop a, int b) {
     ++flopcount::g_tally.pow;
     return flopcount::Flop(std::pow(a.v, b));
 }
-}  // namespace std
+}  // namespace flopstd
 
 using flopcount::Flop;
 
 #define double Flop
+#define std flopstd
 #include "rh_oracle.cpp"
+#undef std
 #undef double
 
 extern "C" {
