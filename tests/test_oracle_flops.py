@@ -4,10 +4,9 @@ FP64 type).  The counted run must compute exactly what the plain oracle
 computes; the counts it reports are the arithmetic the measurement divides
 the GPU's throughput by.
 
-Pins: six Riemann solves per cell-update with two sound speeds each give
-exactly 12 square roots per cell for LLF, HLL and HLLC (the survey's gprof
-profile counts 6 solve_hllc calls per cell-update); a state at rest takes the
-same operations as a moving one up to branch-dependent work."""
+Pin: six Riemann solves per cell-update with two sound speeds each give
+exactly 12 square roots per cell for LLF, HLL and HLLC (a gprof profile of
+the reference binary counts 6 solve_hllc calls per cell-update)."""
 import numpy as np
 import pytest
 

@@ -1,7 +1,7 @@
 """The oracle against facts recorded from running the reference itself on the
-Sedov blast of benchmark config 1 (64^3 cells = reference level 7; E=1 as
+64^3 Sedov blast (64^3 cells = reference level 7; E=1 as
 pressure in the 8 centre cells; HLLC, MinMod, CFL 0.8).  The values were
-logged by a run of the reference's own binary (the survey's probe): they pin
+logged by a probe run of the reference's own binary: they pin
 the composite godunov_fine semantics -- ghost branch on every face, refluxing
 after the clamp in processing order, the Courant pressure cap -- because any
 slip in those changes them.
