@@ -472,12 +472,12 @@ class HostPool {
 
 // Runs fn(i) for i in [0, n) on the host's cores (the flag passes touch each
 // oct's own cells only, or defer their writes, so octs are independent).
-template <class Fn> void parallel_for(int n, Fn fn) {
+template <class Fn> void parallel_for(int n, Fn fn, int min_parallel = 4096) {
     static const int nthreads = [] {
         unsigned h = std::thread::hardware_concurrency();
         return (int)std::max(1u, std::min(h == 0 ? 1u : h, 16u));
     }();
-    if (n < 4096 || nthreads == 1) {
+    if (n < min_parallel || nthreads == 1) {
         for (int i = 0; i < n; ++i) fn(i);
         return;
     }
@@ -752,11 +752,26 @@ int make_grid_fine(rh_amr* a, int il) {
         for (int ic = 1; ic <= a->ncoarse; ++ic)
             if (a->flag1[ic - 1] == 1 && a->son[ic - 1] == 0) cells.push_back(ic);
     } else {
-        for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1])
-            for (int ic = 1; ic <= 8; ++ic) {
-                int idc = cell(a, ig, ic);
-                if (a->flag1[idc - 1] == 1 && a->son[idc - 1] == 0) cells.push_back(idc);
-            }
+        // Flagged leaf cells in list order (oct by oct, cell by cell), found
+        // by the worker threads on long lists (a short scan costs less than
+        // waking them) and gathered in that order.
+        const std::vector<int> grids = list_grids(a, il);
+        std::vector<unsigned char> hit(grids.size(), 0);
+        parallel_for(
+            (int)grids.size(),
+            [&](int k) {
+                unsigned char bits = 0;
+                for (int ic = 1; ic <= 8; ++ic) {
+                    const int idc = cell(a, grids[k], ic);
+                    if (a->flag1[idc - 1] == 1 && a->son[idc - 1] == 0) bits |= (unsigned char)(1u << (ic - 1));
+                }
+                hit[k] = bits;
+            },
+            1 << 17);
+        for (size_t k = 0; k < grids.size(); ++k)
+            if (hit[k])
+                for (int ic = 1; ic <= 8; ++ic)
+                    if ((hit[k] >> (ic - 1)) & 1) cells.push_back(cell(a, grids[k], ic));
     }
     if (cells.empty()) return RH_OK;
     std::vector<int> idx;
