@@ -71,6 +71,10 @@ struct rh_amr {
     int ncells_cap = 0;
     unsigned char* d_flags = nullptr;
     double* d_times = nullptr;  // rh::AMR_T_SIZE doubles
+    // Each level list's grids in list order, and the edit count of the list
+    // they were read at (list_version is bumped by every edit of the list).
+    std::vector<std::vector<int>> list_cache;
+    std::vector<long long> list_cache_version, list_version;
     bool topo_dirty = true;
     long long epoch = 1;
     rh_mesh mesh{};
@@ -156,6 +160,7 @@ void tree_changed(rh_amr* a, int k) {
     std::vector<rhi::DevLevel>& levels = a->ctx->levels;
     for (int il = 1; il < (int)levels.size(); ++il)
         if (levels[il].epoch == old && (il < k - 1 || il > k + 1)) levels[il].epoch = a->epoch;
+    if (k >= 0 && k < (int)a->list_version.size()) a->list_version[k]++;
 }
 
 // The whole host father/nbor once, before the first kernel that reads them
@@ -194,6 +199,9 @@ void allocate(rh_amr* a) {
     a->headl.assign(nl, 0);
     a->taill.assign(nl, 0);
     a->numbl.assign(nl, 0);
+    a->list_cache.assign(nl, std::vector<int>());
+    a->list_cache_version.assign(nl, -1);
+    a->list_version.assign(nl, 0);
     a->next.assign(a->ngridmax, 0);
     a->prev.assign(a->ngridmax, 0);
     a->father.assign(a->ngridmax, 0);
@@ -492,12 +500,18 @@ template <class Fn> void parallel_for(int n, Fn fn, int min_parallel = 4096) {
     pool->run(nchunks, body);
 }
 
-// The grids of a level list, in list order.
-std::vector<int> list_grids(const rh_amr* a, int il) {
-    std::vector<int> g;
-    g.reserve(a->numbl[il]);
-    for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1]) g.push_back(ig);
-    return g;
+// The grids of a level list, in list order: walked again only after the
+// list was edited (every edit goes through tree_changed).  The reference
+// stays valid until the list's next edit.
+const std::vector<int>& list_grids(rh_amr* a, int il) {
+    if (a->list_cache_version[il] != a->list_version[il]) {
+        std::vector<int>& g = a->list_cache[il];
+        g.clear();
+        g.reserve(a->numbl[il]);
+        for (int ig = a->headl[il]; ig > 0; ig = a->next[ig - 1]) g.push_back(ig);
+        a->list_cache_version[il] = a->list_version[il];
+    }
+    return a->list_cache[il];
 }
 
 // ---- tree updater (TreeUpdater.cpp) ---------------------------------------
@@ -722,7 +736,8 @@ int flag_fine(rh_amr* a, int il, int icount, int nsub) {
     if (il >= a->nlevelmax) return RH_OK;
     const int lmin = a->levelmin;
     // The list does not change during the passes (they only set flags).
-    const std::vector<int> grids = il == 1 ? std::vector<int>() : list_grids(a, il);
+    static const std::vector<int> no_grids;
+    const std::vector<int>& grids = il == 1 ? no_grids : list_grids(a, il);
     if (il == 1) {
         for (int i = 0; i < a->ncoarse; ++i) a->flag1[i] = 0;
         for (int i = 1; i <= a->ncoarse; ++i)
@@ -755,7 +770,7 @@ int make_grid_fine(rh_amr* a, int il) {
         // Flagged leaf cells in list order (oct by oct, cell by cell), found
         // by the worker threads on long lists (a short scan costs less than
         // waking them) and gathered in that order.
-        const std::vector<int> grids = list_grids(a, il);
+        const std::vector<int>& grids = list_grids(a, il);
         std::vector<unsigned char> hit(grids.size(), 0);
         parallel_for(
             (int)grids.size(),
@@ -778,7 +793,10 @@ int make_grid_fine(rh_amr* a, int il) {
     idx.reserve(cells.size() * 8);
     for (int ic_coarse : cells) {
         int new_ig = get_free_grid(a);
-        if (new_ig == 0) return afail(a, RH_ERR_ARG, "ngridmax exhausted (the run does not resize)");
+        if (new_ig == 0) {
+            tree_changed(a, il + 1);  // the octs made so far are in the list
+            return afail(a, RH_ERR_ARG, "ngridmax exhausted (the run does not resize)");
+        }
         a->son[ic_coarse - 1] = new_ig;
         a->father[new_ig - 1] = ic_coarse;
         int icn[6];
