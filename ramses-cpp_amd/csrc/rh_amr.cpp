@@ -841,31 +841,37 @@ int make_grid_fine(rh_amr* a, int il) {
 // TreeUpdater.cpp:152-193
 void remove_grid_fine(rh_amr* a, int il) {
     if (il >= a->nlevelmax) return;
-    bool changed = false;
-    for (int ig = a->headl[il]; ig > 0;) {
-        int nxt = a->next[ig - 1];
-        int fc = a->father[ig - 1];
-        bool remove = false;
-        if (fc > 0 && a->flag1[fc - 1] == 0) {
-            remove = true;
+    // Whether an oct goes depends only on its father's flag and its own
+    // cells' sons, which no removal from this list changes (a removal clears
+    // the removed oct's cells and its father's son link): decided on the
+    // worker threads for long lists, then the octs are unlinked in list
+    // order.  The cached list stays as it is until tree_changed.
+    const std::vector<int>& grids = list_grids(a, il);
+    std::vector<unsigned char> drop(grids.size(), 0);
+    parallel_for(
+        (int)grids.size(),
+        [&](int k) {
+            const int ig = grids[k];
+            const int fc = a->father[ig - 1];
+            if (fc <= 0 || a->flag1[fc - 1] != 0) return;
             for (int ic = 1; ic <= 8; ++ic)
-                if (a->son[cell(a, ig, ic) - 1] > 0) {
-                    remove = false;
-                    break;
-                }
-        }
-        if (remove) {
-            for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
-            int p = a->prev[ig - 1], n = a->next[ig - 1];
-            if (p > 0) a->next[p - 1] = n;
-            else a->headl[il] = n;
-            if (n > 0) a->prev[n - 1] = p;
-            else a->taill[il] = p;
-            a->numbl[il]--;
-            free_grid(a, ig);
-            changed = true;
-        }
-        ig = nxt;
+                if (a->son[cell(a, ig, ic) - 1] > 0) return;
+            drop[k] = 1;
+        },
+        1 << 17);
+    bool changed = false;
+    for (size_t k = 0; k < grids.size(); ++k) {
+        if (!drop[k]) continue;
+        const int ig = grids[k];
+        for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
+        int p = a->prev[ig - 1], n = a->next[ig - 1];
+        if (p > 0) a->next[p - 1] = n;
+        else a->headl[il] = n;
+        if (n > 0) a->prev[n - 1] = p;
+        else a->taill[il] = p;
+        a->numbl[il]--;
+        free_grid(a, ig);
+        changed = true;
     }
     if (changed) tree_changed(a, il);
 }
