@@ -54,28 +54,36 @@ template <typename T> RH_DEV void copy5(const T a[5], T b[5]) {
 constexpr int SBLOCK = 256;
 inline int stream_blocks(int noct) { return (noct + SBLOCK - 1) / SBLOCK; }
 
-__global__ __launch_bounds__(SBLOCK) void k_set_unew(MeshDev m, LevelDev lv) {
-    const int p = blockIdx.x * SBLOCK + threadIdx.x;
-    if (p >= lv.noct) return;
-    const int g = lv.grid[p];
+// Copies of an oct's cells (mask bit ic0: cell ic0 is copied) from src to
+// dst, which never overlap: all 40 loads are issued before the first store,
+// so a short level costs one memory round trip per thread, not eight.
+RH_DEV void copy_oct(const MeshDev& m, int g, int mask, const double* __restrict__ src, double* __restrict__ dst) {
+    double v[8][5];
+#pragma unroll
     for (int ic0 = 0; ic0 < 8; ++ic0) {
         const size_t c = (size_t)cell_id(m, g, ic0) - 1;
 #pragma unroll
-        for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = m.uold[(size_t)iv * m.ncell + c];
+        for (int iv = 0; iv < 5; ++iv) v[ic0][iv] = ((mask >> ic0) & 1) ? src[(size_t)iv * m.ncell + c] : 0.0;
     }
+#pragma unroll
+    for (int ic0 = 0; ic0 < 8; ++ic0) {
+        if (!((mask >> ic0) & 1)) continue;
+        const size_t c = (size_t)cell_id(m, g, ic0) - 1;
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) dst[(size_t)iv * m.ncell + c] = v[ic0][iv];
+    }
+}
+
+__global__ __launch_bounds__(SBLOCK) void k_set_unew(MeshDev m, LevelDev lv) {
+    const int p = blockIdx.x * SBLOCK + threadIdx.x;
+    if (p >= lv.noct) return;
+    copy_oct(m, lv.grid[p], 0xff, m.uold, m.unew);
 }
 
 __global__ __launch_bounds__(SBLOCK) void k_set_uold(MeshDev m, LevelDev lv) {
     const int p = blockIdx.x * SBLOCK + threadIdx.x;
     if (p >= lv.noct) return;
-    const int g = lv.grid[p];
-    const int lm = lv.leaf[p];
-    for (int ic0 = 0; ic0 < 8; ++ic0) {
-        if (!((lm >> ic0) & 1)) continue;
-        const size_t c = (size_t)cell_id(m, g, ic0) - 1;
-#pragma unroll
-        for (int iv = 0; iv < 5; ++iv) m.uold[(size_t)iv * m.ncell + c] = m.unew[(size_t)iv * m.ncell + c];
-    }
+    copy_oct(m, lv.grid[p], lv.leaf[p], m.unew, m.uold);
 }
 
 RH_DEV double fval(const MeshDev& m, size_t c, int idim) {
@@ -220,18 +228,31 @@ __global__ __launch_bounds__(SBLOCK) void k_courant_partial(MeshDev m, LevelDev 
     if (p < lv.noct) {
         const int g = lv.grid[p];
         const int lm = lv.leaf[p];
-        for (int ic0 = 0; ic0 < 8; ++ic0) {
-            const int id = cell_id(m, g, ic0);
-            if constexpr (SET_UNEW) {
-                double u[5];
-                load_u(m.uold, m.ncell, id, u);
-                const size_t c = (size_t)id - 1;
+        if constexpr (SET_UNEW) {
+            // All eight cells loaded before the copies are stored (uold and
+            // unew never overlap), then the candidates of the leaf cells.
+            double v[8][5];
+            const double* __restrict__ src = m.uold;
+            double* __restrict__ dst = m.unew;
 #pragma unroll
-                for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = u[iv];
-                if ((lm >> ic0) & 1) cand = min_skip_nan(cand, courant_cand_u(m, u, id, dx, gamma, courant_factor));
-            } else if ((lm >> ic0) & 1) {
-                cand = min_skip_nan(cand, courant_cand(m, id, dx, gamma, courant_factor));
+            for (int ic0 = 0; ic0 < 8; ++ic0) {
+                const size_t c = (size_t)cell_id(m, g, ic0) - 1;
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) v[ic0][iv] = src[(size_t)iv * m.ncell + c];
             }
+#pragma unroll
+            for (int ic0 = 0; ic0 < 8; ++ic0) {
+                const size_t c = (size_t)cell_id(m, g, ic0) - 1;
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) dst[(size_t)iv * m.ncell + c] = v[ic0][iv];
+            }
+#pragma unroll
+            for (int ic0 = 0; ic0 < 8; ++ic0)
+                if ((lm >> ic0) & 1)
+                    cand = min_skip_nan(cand, courant_cand_u(m, v[ic0], cell_id(m, g, ic0), dx, gamma, courant_factor));
+        } else {
+            for (int ic0 = 0; ic0 < 8; ++ic0)
+                if ((lm >> ic0) & 1) cand = min_skip_nan(cand, courant_cand(m, cell_id(m, g, ic0), dx, gamma, courant_factor));
         }
     }
     block_min(cand, part);
