@@ -1016,11 +1016,16 @@ int amr_step(rh_amr* a, int il, int icount) {
         RH_ACHK(rh_level_leaf_cells(a->ctx, il, &nleaf));
         a->cell_updates += nleaf;
         RH_ACHK(godunov_dev(a->ctx, il, a->d_times + rh::AMR_T_USED + il, dx));
-        RH_ACHK(rh_set_uold(a->ctx, il));
     }
-    if (il < a->nlevelmax) {
+    if (il < a->nlevelmax && count(a, il + 1) > 0) {
+        // set_uold of this level and restrict_fine of the next one in one
+        // launch (they write disjoint cells of this level).
         Phase ph{a, PH_RESTRICT};
-        RH_ACHK(restrict_fine(a, il));
+        RH_ACHK(upload_topology(a));
+        RH_ACHK(set_uold_restrict_dev(a->ctx, il, a->d_father));
+    } else {
+        Phase ph{a, PH_HYDRO};
+        RH_ACHK(rh_set_uold(a->ctx, il));
     }
     Phase ph{a, PH_FLAG};
     return flag_fine(a, il + 1, icount, il > 0 ? a->nsubcycle[il - 1] : 1);

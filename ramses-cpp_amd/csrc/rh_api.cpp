@@ -289,6 +289,22 @@ int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx) {
     return do_godunov(c, lv, d_dt, dx);
 }
 
+int set_uold_restrict_dev(rh_ctx* c, int ilevel, const int* d_father) {
+    int st = check_bound(c);
+    if (st) return st;
+    if (c->mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "the fused restriction needs a device-resident mesh");
+    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+    // The finer level first: get_level may grow the level vector, which the
+    // second call then leaves alone, so both pointers stay valid.
+    const rh::LevelDev *lv, *fine;
+    if ((st = get_level(c, ilevel + 1, &fine))) return st;
+    if ((st = get_level(c, ilevel, &lv))) return st;
+    Timed tm(c, TK_SET_UOLD);
+    RH_HIP(c, rh::launch_set_uold_restrict(mesh_dev(c), *lv, *fine, d_father, c->stream));
+    tm.done();
+    return RH_OK;
+}
+
 }  // namespace rhi
 
 using namespace rhi;

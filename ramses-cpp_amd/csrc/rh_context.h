@@ -104,6 +104,10 @@ int pre_call(rh_ctx* c);
 int courant_dev(rh_ctx* c, int ilevel, double dx, double gamma, double courant_factor, double* d_out,
                 bool set_unew = false, const rh::AmrTimesArgs& at = rh::AmrTimesArgs{});
 int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx);
+// rh_set_uold(ilevel) followed by the restriction of level ilevel+1 into
+// its father cells (d_father: father cell of every grid, device memory), in
+// one launch; device-resident meshes only.
+int set_uold_restrict_dev(rh_ctx* c, int ilevel, const int* d_father);
 // Frees every device buffer of the current binding (single or ensemble).
 void unbind(rh_ctx* c);
 

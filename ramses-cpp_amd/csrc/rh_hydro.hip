@@ -1,20 +1,35 @@
 // rh_hydro.hip -- gfx950 kernels of the per-level Godunov hydro path.
 //
-// Kernels (one level per launch):
-//   k_set_unew      unew <- uold for every cell of the level's octs
-//   k_set_uold      uold <- unew for the level's leaf cells
-//   k_courant_*     minimum Courant time step over the level's leaf cells
-//                   (wavefront shuffles, then one block-level pass)
-//   k_godunov       fluxes, ordered reflux accumulation and clamps
-//   k_coarse_reflux ordered reflux into coarser leaf cells (AMR faces)
-//   k_interpol      prolongation of coarse cells into new octs
+// Kernels (one level per launch unless stated):
+//   k_set_unew / k_set_uold   unew <- uold for every cell of the level's octs /
+//                             uold <- unew for its leaf cells
+//   k_courant_partial/_final  minimum Courant time step over the level's leaf
+//                             cells (wavefront shuffles, block minima, then one
+//                             1024-thread pass; the first can also do set_unew)
+//   k_step_join               set_uold, the next Courant candidates and
+//                             set_unew in one pass between two steps
+//   k_courant_boxes           the Courant minimum per box of an ensemble (64-bit atomicMin
+//                             on an order-preserving key of the double;
+//                             k_fill_keys / k_keys_to_dt around it)
+//   k_godunov                 per-cell form: fluxes, ordered reflux
+//                             accumulation and clamps
+//   k_godunov_oct             oct-in-lanes form of the same update (siblings by
+//                             cross-lane shuffle, ordered accumulation in LDS)
+//   k_coarse_reflux(_wave)    ordered reflux into coarser leaf cells
+//   k_interpol / k_new_octs   prolongation of coarse cells into new octs
+//   k_restrict, k_set_uold_restrict, k_grad_flags, k_amr_times
+//                             adaptive-run restriction, gradient flags and
+//                             subcycling time-step bookkeeping
+//   k_gravity_source / k_synchro, k_pack_hydro
+//                             gravity kicks; snapshot packing
 //
-// The Godunov update is gather-only and deterministic: every leaf cell owns
-// its unew entry and rebuilds, in registers, every contribution the
+// The Godunov update is deterministic and uses no floating-point atomics:
+// every leaf cell owns its unew entry and gathers every contribution the
 // reference's sequential loop adds to it -- its own six face fluxes and the
 // six fluxes its same-level neighbours compute for the faces they share --
 // then adds them in the reference's processing order (oct position in the
-// level list x 8 + cell, then face).  No atomics, no inter-thread exchange.
+// level list x 8 + cell, then face).  Contributions to coarser cells are
+// collected per target and subtracted in the same order by k_coarse_reflux.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -1334,10 +1349,8 @@ __global__ __launch_bounds__(64) void k_amr_times(double* T, int op, int il, int
 
 // restrict_fine (TreeUpdater.cpp:195-215): the father cell of every oct of
 // the finer list gets the mean of its eight children's uold.
-__global__ __launch_bounds__(256) void k_restrict(MeshDev m, LevelDev fine, const int* __restrict__ father) {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= fine.noct) return;
-    const int g = fine.grid[p];
+// Mean of the oct's eight children, summed in cell order, into its father cell.
+RH_DEV void restrict_oct(const MeshDev& m, int g, const int* __restrict__ father) {
     const int fc = father[g - 1];
     if (fc <= 0) return;
     for (int iv = 0; iv < 5; ++iv) {
@@ -1345,6 +1358,32 @@ __global__ __launch_bounds__(256) void k_restrict(MeshDev m, LevelDev fine, cons
         for (int ic = 0; ic < 8; ++ic) sum += m.uold[(size_t)iv * m.ncell + (size_t)cell_id(m, g, ic) - 1];
         m.uold[(size_t)iv * m.ncell + (size_t)fc - 1] = sum / 8;
     }
+}
+
+__global__ __launch_bounds__(256) void k_restrict(MeshDev m, LevelDev fine, const int* __restrict__ father) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= fine.noct) return;
+    restrict_oct(m, fine.grid[p], father);
+}
+
+// set_uold of one level and the restriction of the next finer level into it
+// (the two calls that end a level step of an adaptive run) in one launch:
+// blocks [0, nb_set) copy the level's leaf cells as k_set_uold does, the
+// remaining blocks average the finer octs into their father cells as
+// k_restrict does.  The first part writes leaf cells of the level from unew,
+// the second refined cells of the level from the finer level's uold, so the
+// two touch disjoint cells and need no ordering between them.
+__global__ __launch_bounds__(SBLOCK) void k_set_uold_restrict(MeshDev m, LevelDev lv, LevelDev fine,
+                                                              const int* __restrict__ father, int nb_set) {
+    if ((int)blockIdx.x < nb_set) {
+        const int p = blockIdx.x * SBLOCK + threadIdx.x;
+        if (p >= lv.noct) return;
+        copy_oct(m, lv.grid[p], lv.leaf[p], m.unew, m.uold);
+        return;
+    }
+    const int p = ((int)blockIdx.x - nb_set) * SBLOCK + threadIdx.x;
+    if (p >= fine.noct) return;
+    restrict_oct(m, fine.grid[p], father);
 }
 
 struct FlagState {
@@ -1647,6 +1686,15 @@ hipError_t launch_amr_times(double* d_T, int op, int il, int lmin, double nsub_p
 hipError_t launch_restrict(const MeshDev& m, const LevelDev& fine, const int* d_father, hipStream_t stream) {
     if (fine.noct == 0) return hipSuccess;
     hipLaunchKernelGGL(k_restrict, dim3((fine.noct + 255) / 256), dim3(256), 0, stream, m, fine, d_father);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_uold_restrict(const MeshDev& m, const LevelDev& lv, const LevelDev& fine, const int* d_father,
+                                    hipStream_t stream) {
+    const int nb_set = stream_blocks(lv.noct), nb_fine = stream_blocks(fine.noct);
+    if (nb_set + nb_fine == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_uold_restrict, dim3(nb_set + nb_fine), dim3(SBLOCK), 0, stream, m, lv, fine, d_father,
+                       nb_set);
     return hipGetLastError();
 }
 

@@ -123,6 +123,9 @@ hipError_t launch_keys_to_dt(const unsigned long long* keys, int n, double* dt, 
 hipError_t launch_new_octs(const MeshDev& m, int n, const int* d_cells, const GasParams& gp, int interpol_var,
                            int* d_father, int* d_nbor, hipStream_t stream);
 hipError_t launch_restrict(const MeshDev& m, const LevelDev& fine, const int* d_father, hipStream_t stream);
+// set_uold of lv and the restriction of fine (the next finer level) into it, one launch.
+hipError_t launch_set_uold_restrict(const MeshDev& m, const LevelDev& lv, const LevelDev& fine, const int* d_father,
+                                    hipStream_t stream);
 // Adaptive run's time-step table on the device (doubles): dtnew, dtold and
 // the dt each level's last step used, AMR_T_LEVELS entries each, then the
 // time t and the latest Courant minimum.
