@@ -25,7 +25,9 @@ typedef enum {
     RH_ERR_HIP = 2,          /* HIP runtime error (message has details)   */
     RH_ERR_UNSUPPORTED = 3,  /* configuration outside the GPU path        */
     RH_ERR_TOPOLOGY = 4,     /* mesh the reference itself cannot produce  */
-    RH_ERR_NO_DEVICE = 5     /* no usable GPU                             */
+    RH_ERR_NO_DEVICE = 5,    /* no usable GPU                             */
+    RH_ERR_HOST = 6          /* host-side failure: memory exhausted or an
+                                internal exception (message has details) */
 } rh_status;
 
 /* Riemann solver ids.  The adapter resolves the namelist string exactly as

@@ -54,33 +54,47 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        L.or_godunov_fine.argtypes = [C.POINTER(OrMesh), C.POINTER(OrParams), C.c_int, C.c_double, C.c_double]
-        L.or_godunov_fine_sample.argtypes = [C.POINTER(OrMesh), C.POINTER(OrParams), C.c_int, C.c_double,
-                                             C.c_double, C.c_longlong]
-        L.or_set_unew.argtypes = [C.POINTER(OrMesh), C.c_int]
-        L.or_set_uold.argtypes = [C.POINTER(OrMesh), C.c_int]
-        L.or_courant.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
-        L.or_courant.restype = C.c_double
-        L.or_add_gravity_source_terms.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double]
-        L.or_synchro_hydro_fine.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double, C.c_double]
-        L.or_interpol_hydro.argtypes = [C.POINTER(OrParams), C.c_int, _DP, _DP]
-        L.or_ctoprim.argtypes = [_DP, _DP, C.c_double]
-        L.or_trace.argtypes = [_DP, _DP, C.c_double, C.c_double, _DP, _DP, C.c_double]
-        L.or_compute_slope.argtypes = [C.c_double] * 3 + [C.c_int, C.c_double, C.c_double]
-        L.or_compute_slope.restype = C.c_double
-        L.or_ghost_slope.argtypes = [C.c_double, C.c_double, C.c_int]
-        L.or_ghost_slope.restype = C.c_double
-        L.or_get_cs2.argtypes = [C.c_double] * 3
-        L.or_get_cs2.restype = C.c_double
-        L.or_riemann.argtypes = [C.c_int, _DP, _DP, _DP, C.c_double]
-        L.or_prim_to_cons.argtypes = [_DP, _DP, C.c_double]
-        L.or_compute_flux.argtypes = [_DP, _DP, C.c_double]
-        L.or_build_uniform.argtypes = [C.c_int] * 6 + [_IP] * 8
-        L.or_refine.argtypes = [C.POINTER(OrMesh), C.POINTER(OrParams), C.c_int, _IP, C.c_int, _IP, _IP, _IP,
-                                _DP, C.POINTER(C.c_int), _IP, _IP, _IP, _IP, _IP]
-        _lib = L
+        _lib = _configure(C.CDLL(LIB_PATH))
     return _lib
+
+
+def planted(plant: int, outdir: str) -> C.CDLL:
+    """The oracle compiled with planted mistake `plant` (RH_ORACLE_PLANT in
+    rh_oracle.cpp) into outdir, for tests that must show a check fails on a
+    wrong restatement.  Same flags as the Makefile."""
+    path = os.path.join(outdir, f"liboracle_plant{plant}.so")
+    srcs = [os.path.join(HERE, f) for f in ("rh_oracle.cpp", "rh_oracle_amr.cpp", "rh_oracle_snapshot.cpp")]
+    subprocess.run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+                    f"-DRH_ORACLE_PLANT={int(plant)}", "-o", path, *srcs], check=True, capture_output=True)
+    return _configure(C.CDLL(path))
+
+
+def _configure(L: C.CDLL) -> C.CDLL:
+    L.or_godunov_fine.argtypes = [C.POINTER(OrMesh), C.POINTER(OrParams), C.c_int, C.c_double, C.c_double]
+    L.or_godunov_fine_sample.argtypes = [C.POINTER(OrMesh), C.POINTER(OrParams), C.c_int, C.c_double,
+                                         C.c_double, C.c_longlong]
+    L.or_set_unew.argtypes = [C.POINTER(OrMesh), C.c_int]
+    L.or_set_uold.argtypes = [C.POINTER(OrMesh), C.c_int]
+    L.or_courant.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.or_courant.restype = C.c_double
+    L.or_add_gravity_source_terms.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double]
+    L.or_synchro_hydro_fine.argtypes = [C.POINTER(OrMesh), C.c_int, C.c_double, C.c_double]
+    L.or_interpol_hydro.argtypes = [C.POINTER(OrParams), C.c_int, _DP, _DP]
+    L.or_ctoprim.argtypes = [_DP, _DP, C.c_double]
+    L.or_trace.argtypes = [_DP, _DP, C.c_double, C.c_double, _DP, _DP, C.c_double]
+    L.or_compute_slope.argtypes = [C.c_double] * 3 + [C.c_int, C.c_double, C.c_double]
+    L.or_compute_slope.restype = C.c_double
+    L.or_ghost_slope.argtypes = [C.c_double, C.c_double, C.c_int]
+    L.or_ghost_slope.restype = C.c_double
+    L.or_get_cs2.argtypes = [C.c_double] * 3
+    L.or_get_cs2.restype = C.c_double
+    L.or_riemann.argtypes = [C.c_int, _DP, _DP, _DP, C.c_double]
+    L.or_prim_to_cons.argtypes = [_DP, _DP, C.c_double]
+    L.or_compute_flux.argtypes = [_DP, _DP, C.c_double]
+    L.or_build_uniform.argtypes = [C.c_int] * 6 + [_IP] * 8
+    L.or_refine.argtypes = [C.POINTER(OrMesh), C.POINTER(OrParams), C.c_int, _IP, C.c_int, _IP, _IP, _IP,
+                            _DP, C.POINTER(C.c_int), _IP, _IP, _IP, _IP, _IP]
+    return L
 
 
 def ref() -> C.CDLL | None:
@@ -117,8 +131,9 @@ class OracleMesh:
     """Oracle view over AmrGrid-shaped numpy arrays (any object exposing
     father/son/nbor/next/headl/uold/unew and the scalars)."""
 
-    def __init__(self, m, f: np.ndarray | None = None):
+    def __init__(self, m, f: np.ndarray | None = None, L: C.CDLL | None = None):
         self.m = m
+        self._L = L
         self._bt = np.zeros(1, np.int32)
         s = OrMesh()
         s.ncoarse, s.ngridmax, s.ncell, s.nvar = m.ncoarse, m.ngridmax, m.ncell, m.nvar
@@ -132,30 +147,33 @@ class OracleMesh:
         s.bound_type, s.nbound_type = _ip(self._bt), 0
         self.s = s
 
+    def lib(self) -> C.CDLL:
+        return self._L if self._L is not None else lib()
+
     def godunov_fine(self, p: OrParams, ilevel, dt, dx):
-        st = lib().or_godunov_fine(C.byref(self.s), C.byref(p), ilevel, dt, dx)
+        st = self.lib().or_godunov_fine(C.byref(self.s), C.byref(p), ilevel, dt, dx)
         if st != 0:
             raise RuntimeError(f"oracle godunov_fine status {st}")
 
     def godunov_fine_sample(self, p: OrParams, ilevel, dt, dx, max_octs):
-        st = lib().or_godunov_fine_sample(C.byref(self.s), C.byref(p), ilevel, dt, dx, max_octs)
+        st = self.lib().or_godunov_fine_sample(C.byref(self.s), C.byref(p), ilevel, dt, dx, max_octs)
         if st != 0:
             raise RuntimeError(f"oracle godunov_fine status {st}")
 
     def set_unew(self, ilevel):
-        lib().or_set_unew(C.byref(self.s), ilevel)
+        self.lib().or_set_unew(C.byref(self.s), ilevel)
 
     def set_uold(self, ilevel):
-        lib().or_set_uold(C.byref(self.s), ilevel)
+        self.lib().or_set_uold(C.byref(self.s), ilevel)
 
     def add_gravity_source_terms(self, ilevel, dt):
-        lib().or_add_gravity_source_terms(C.byref(self.s), ilevel, dt)
+        self.lib().or_add_gravity_source_terms(C.byref(self.s), ilevel, dt)
 
     def synchro_hydro_fine(self, ilevel, dt, gamma):
-        lib().or_synchro_hydro_fine(C.byref(self.s), ilevel, dt, gamma)
+        self.lib().or_synchro_hydro_fine(C.byref(self.s), ilevel, dt, gamma)
 
     def courant(self, ilevel, dx, gamma, courant_factor, boxlen=1.0) -> float:
-        return lib().or_courant(C.byref(self.s), ilevel, dx, gamma, courant_factor, boxlen)
+        return self.lib().or_courant(C.byref(self.s), ilevel, dx, gamma, courant_factor, boxlen)
 
     def run_uniform(self, p: OrParams, ilevel, dx, courant_factor, nsteps) -> np.ndarray:
         """Simulation::amr_step(ilevel, 1) with levelmin == levelmax

@@ -30,6 +30,20 @@
 
 #include "rh_oracle_core.h"
 
+// Planted mistakes, for the tests that show the composite checks can fail
+// (tests/test_oracle_composite.py builds the oracle once per value into a
+// temporary directory).  0, the default and the only value the Makefile
+// builds, is the faithful restatement.
+//   1  the 1/8 reflux into the neighbour is dropped
+//   2  the reflux has the wrong sign
+//   3  refluxes land before the cell's clamp (the clamps run after the
+//      whole level instead of at each cell's turn)
+//   4  the reflux factor is 1/4 instead of 1/8
+//   5  the two face states of every face are swapped (transposed operands)
+#ifndef RH_ORACLE_PLANT
+#define RH_ORACLE_PLANT 0
+#endif
+
 namespace {
 
 // HydroSolver.cpp:613-651 (nener=0, barotropic_eos off)
@@ -597,12 +611,15 @@ struct Solver {
                         swapd(qlf[1], qlf[1 + idim]);
                         swapd(qrf[1], qrf[1 + idim]);
                     }
+                    if (RH_ORACLE_PLANT == 5)
+                        for (int iv = 0; iv < NV; ++iv) swapd(qlf[iv], qrf[iv]);
                     riemann(c.p->riemann, qlf, qrf, flux, gamma);
                     if (idim > 0) swapd(flux[1], flux[1 + idim]);
                     // Refluxing, HydroSolver.cpp:476-482
-                    if (id_n > 0 && cell_levels[id_n] < ilevel) {
-                        double factor = 1.0 / (1 << 3);
+                    if (id_n > 0 && cell_levels[id_n] < ilevel && RH_ORACLE_PLANT != 1) {
+                        double factor = RH_ORACLE_PLANT == 4 ? 1.0 / (1 << 2) : 1.0 / (1 << 3);
                         double sign = (side == 0) ? 1.0 : -1.0;
+                        if (RH_ORACLE_PLANT == 2) sign = -sign;
                         for (int iv = 1; iv <= NV; ++iv) c.unew(id_n, iv) -= dtdx * sign * flux[iv - 1] * factor;
                     }
                 }
@@ -612,7 +629,11 @@ struct Solver {
             }
         }
         for (int iv = 1; iv <= NV; ++iv) c.unew(idc0 + 1, iv) = c.unew(idc0 + 1, iv) + dtdx * flux_sum[iv - 1];
-        // Clamps, HydroSolver.cpp:534-547
+        if (RH_ORACLE_PLANT != 3) clamp_cell(idc0);
+    }
+
+    // Clamps, HydroSolver.cpp:534-547
+    void clamp_cell(int idc0) {
         double d_curr = sclamp(c.unew(idc0 + 1, 1), 1e-10, 1e6);
         c.unew(idc0 + 1, 1) = d_curr;
         for (int i = 1; i <= 3; ++i)
@@ -672,6 +693,12 @@ struct Solver {
                 compute_fluxes(idc0, icn, (int)pos * 8 + ic - 1);
             }
         }
+        if (RH_ORACLE_PLANT == 3)
+            for (size_t pos = 0; pos < n; ++pos)
+                for (int ic = 1; ic <= 8; ++ic) {
+                    int idc0 = m->ncoarse + (ic - 1) * m->ngridmax + octs[pos] - 1;
+                    if (m->son[idc0] == 0) clamp_cell(idc0);
+                }
         return error;
     }
 };

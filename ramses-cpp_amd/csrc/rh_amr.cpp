@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "rh_context.h"
+#include "rh_guard.h"
 #include "rh_kernels.h"
 #include "rh_tables.h"
 #include "rhydro.h"
@@ -1063,146 +1064,160 @@ void destroy(rh_amr* a) {
 extern "C" {
 
 int rh_amr_create(const rh_params* hydro, const rh_amr_params* ap, rh_amr** out) {
-    if (!hydro || !ap || !out) return RH_ERR_ARG;
-    *out = nullptr;
-    if (ap->levelmin < 1 || ap->levelmax < ap->levelmin || ap->levelmax > 30 || ap->ngridmax < 1 || ap->nx < 1 ||
-        ap->ny < 1 || ap->nz < 1 || ap->nexpand < 0 || (ap->ic_kind == 0 && ap->nreg > 0 && !ap->reg_type))
-        return RH_ERR_ARG;
-    const long long ncell = (long long)ap->nx * ap->ny * ap->nz + 8LL * ap->ngridmax;
-    if (ncell > INT_MAX / 5) return RH_ERR_ARG;
-    rh_amr* a = new rh_amr();
-    a->hp = *hydro;
-    a->ap = *ap;
-    a->nx = ap->nx;
-    a->ny = ap->ny;
-    a->nz = ap->nz;
-    a->ngridmax = ap->ngridmax;
-    a->nlevelmax = ap->levelmax;
-    a->levelmin = ap->levelmin;
-    a->boxlen = hydro->boxlen;
-    a->gamma = hydro->gamma;
-    if (ap->ic_kind == 0)
-        for (int r = 0; r < ap->nreg; ++r) {
-            a->reg_type.push_back(ap->reg_type[r]);
-            const double* cols[11] = {ap->x_center, ap->y_center, ap->z_center, ap->length_x, ap->length_y,
-                                      ap->length_z, ap->d_region, ap->u_region, ap->v_region, ap->w_region,
-                                      ap->p_region};
-            for (int k = 0; k < 11; ++k) a->reg[k].push_back(cols[k][r]);
+    return guard(&g_create_error, nullptr, [&]() -> int {
+        if (!hydro || !ap || !out) return RH_ERR_ARG;
+        *out = nullptr;
+        if (ap->levelmin < 1 || ap->levelmax < ap->levelmin || ap->levelmax > 30 || ap->ngridmax < 1 || ap->nx < 1 ||
+            ap->ny < 1 || ap->nz < 1 || ap->nexpand < 0 || (ap->ic_kind == 0 && ap->nreg > 0 && !ap->reg_type))
+            return RH_ERR_ARG;
+        const long long ncell = (long long)ap->nx * ap->ny * ap->nz + 8LL * ap->ngridmax;
+        if (ncell > INT_MAX / 5) return RH_ERR_ARG;
+        rh_amr* a = new rh_amr();
+        a->hp = *hydro;
+        a->ap = *ap;
+        a->nx = ap->nx;
+        a->ny = ap->ny;
+        a->nz = ap->nz;
+        a->ngridmax = ap->ngridmax;
+        a->nlevelmax = ap->levelmax;
+        a->levelmin = ap->levelmin;
+        a->boxlen = hydro->boxlen;
+        a->gamma = hydro->gamma;
+        if (ap->ic_kind == 0)
+            for (int r = 0; r < ap->nreg; ++r) {
+                a->reg_type.push_back(ap->reg_type[r]);
+                const double* cols[11] = {ap->x_center, ap->y_center, ap->z_center, ap->length_x, ap->length_y,
+                                          ap->length_z, ap->d_region, ap->u_region, ap->v_region, ap->w_region,
+                                          ap->p_region};
+                for (int k = 0; k < 11; ++k) a->reg[k].push_back(cols[k][r]);
+            }
+        a->ap.reg_type = nullptr;  // the run keeps its own copy
+        int st = rh_create(hydro, &a->ctx);
+        if (st != RH_OK) {
+            delete a;
+            return st;
         }
-    a->ap.reg_type = nullptr;  // the run keeps its own copy
-    int st = rh_create(hydro, &a->ctx);
-    if (st != RH_OK) {
-        delete a;
-        return st;
-    }
-    allocate(a);
-    refresh_mesh(a);
-    st = rh_bind(a->ctx, &a->mesh, RH_DEVICE_RESIDENT);
-    if (st == RH_OK) {
-        hipError_t e = hipMalloc(&a->d_father, (size_t)a->ngridmax * sizeof(int));
-        if (e == hipSuccess) e = hipMalloc(&a->d_nbor, (size_t)6 * a->ngridmax * sizeof(int));
-        if (e == hipSuccess) e = hipMalloc(&a->d_flags, (size_t)8 * a->ngridmax);
-        if (e == hipSuccess) e = hipMalloc(&a->d_times, (size_t)rh::AMR_T_SIZE * sizeof(double));
-        if (e == hipSuccess) {
-            const std::vector<double> zero(rh::AMR_T_SIZE, 0.0);  // dtnew = dtold = 0, t = 0
-            e = hipMemcpy(a->d_times, zero.data(), zero.size() * sizeof(double), hipMemcpyHostToDevice);
+        allocate(a);
+        refresh_mesh(a);
+        st = rh_bind(a->ctx, &a->mesh, RH_DEVICE_RESIDENT);
+        if (st == RH_OK) {
+            hipError_t e = hipMalloc(&a->d_father, (size_t)a->ngridmax * sizeof(int));
+            if (e == hipSuccess) e = hipMalloc(&a->d_nbor, (size_t)6 * a->ngridmax * sizeof(int));
+            if (e == hipSuccess) e = hipMalloc(&a->d_flags, (size_t)8 * a->ngridmax);
+            if (e == hipSuccess) e = hipMalloc(&a->d_times, (size_t)rh::AMR_T_SIZE * sizeof(double));
+            if (e == hipSuccess) {
+                const std::vector<double> zero(rh::AMR_T_SIZE, 0.0);  // dtnew = dtold = 0, t = 0
+                e = hipMemcpy(a->d_times, zero.data(), zero.size() * sizeof(double), hipMemcpyHostToDevice);
+            }
+            if (e != hipSuccess) st = afail(a, RH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
         }
-        if (e != hipSuccess) st = afail(a, RH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    }
-    if (st == RH_OK) st = initialize(a);
-    if (st != RH_OK) {
-        g_create_error = a->err.empty() ? rh_last_error(a->ctx) : a->err;
-        destroy(a);
-        return st;
-    }
-    *out = a;
-    return RH_OK;
+        if (st == RH_OK) st = initialize(a);
+        if (st != RH_OK) {
+            g_create_error = a->err.empty() ? rh_last_error(a->ctx) : a->err;
+            destroy(a);
+            return st;
+        }
+        *out = a;
+        return RH_OK;
+    });
 }
 
 int rh_amr_destroy(rh_amr* a) {
-    if (a) destroy(a);
-    return RH_OK;
+    return guard(a ? &a->err : &g_create_error, (a && a->ctx) ? &a->ctx->err : nullptr, [&]() -> int {
+        if (a) destroy(a);
+        return RH_OK;
+    });
 }
 
 int rh_amr_step(rh_amr* a, int nsteps) {
-    if (!a || nsteps < 0) return RH_ERR_ARG;
-    RH_AHIP(a, hipSetDevice(a->hp.device));
-    for (int k = 0; k < nsteps; ++k) RH_ACHK(step_coarse(a));
-    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
-    return RH_OK;
+    return guard(a ? &a->err : &g_create_error, (a && a->ctx) ? &a->ctx->err : nullptr, [&]() -> int {
+        if (!a || nsteps < 0) return RH_ERR_ARG;
+        RH_AHIP(a, hipSetDevice(a->hp.device));
+        for (int k = 0; k < nsteps; ++k) RH_ACHK(step_coarse(a));
+        RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
+        return RH_OK;
+    });
 }
 
 int rh_amr_mesh(rh_amr* a, rh_mesh* mesh, const int** numbl, const double** xg) {
-    if (!a) return RH_ERR_ARG;
-    RH_ACHK(rh_sync(a->ctx, RH_SYNC_TO_HOST, RH_UOLD | RH_UNEW));
-    refresh_mesh(a);
-    if (mesh) *mesh = a->mesh;
-    if (numbl) *numbl = a->numbl.data();
-    if (xg) *xg = a->xg.data();
-    return RH_OK;
+    return guard(a ? &a->err : &g_create_error, (a && a->ctx) ? &a->ctx->err : nullptr, [&]() -> int {
+        if (!a) return RH_ERR_ARG;
+        RH_ACHK(rh_sync(a->ctx, RH_SYNC_TO_HOST, RH_UOLD | RH_UNEW));
+        refresh_mesh(a);
+        if (mesh) *mesh = a->mesh;
+        if (numbl) *numbl = a->numbl.data();
+        if (xg) *xg = a->xg.data();
+        return RH_OK;
+    });
 }
 
 int rh_amr_stats(rh_amr* a, double* t, int* nstep, long long* cell_updates) {
-    if (!a) return RH_ERR_ARG;
-    RH_ACHK(sync_times(a));
-    if (t) *t = a->t;
-    if (nstep) *nstep = a->nstep;
-    if (cell_updates) *cell_updates = a->cell_updates;
-    return RH_OK;
+    return guard(a ? &a->err : &g_create_error, (a && a->ctx) ? &a->ctx->err : nullptr, [&]() -> int {
+        if (!a) return RH_ERR_ARG;
+        RH_ACHK(sync_times(a));
+        if (t) *t = a->t;
+        if (nstep) *nstep = a->nstep;
+        if (cell_updates) *cell_updates = a->cell_updates;
+        return RH_OK;
+    });
 }
 
 rh_ctx* rh_amr_context(rh_amr* a) { return a ? a->ctx : nullptr; }
 
 int rh_amr_dump(rh_amr* a, const rh_snapshot_info* info, int iout, const char* root) {
-    if (!a || !root) return RH_ERR_ARG;
-    rh_snapshot_info in{};
-    if (info) {
-        in = *info;
-    } else {  // SnapshotInfo's defaults (RamsesWriter.hpp:24-31), no outputs, unit scales 1
-        in.aexp_ini = in.boxlen_ini = in.aexp = in.aexp_old = 1.0;
-        in.units_length = in.units_density = in.units_time = 1.0;
-    }
-    // What Simulation::dump_snapshot takes from the run (Simulation.cpp:683-692).
-    RH_ACHK(sync_times(a));
-    in.t = a->t;
-    in.nstep = in.nstep_coarse = a->nstep;
-    in.iout = iout;
-    in.levelmin = a->levelmin;
-    in.levelmax = a->nlevelmax;
-    in.boxlen = a->boxlen;
-    rh_snapshot_tree t{};
-    t.nx = a->nx;
-    t.ny = a->ny;
-    t.nz = a->nz;
-    t.ncoarse = a->ncoarse;
-    t.ngridmax = a->ngridmax;
-    t.ncell = a->ncell;
-    t.nlevelmax = a->nlevelmax;
-    t.ncpu = 1;
-    t.nboundary = 0;
-    t.gamma = a->gamma;
-    t.headl = a->headl.data();
-    t.taill = a->taill.data();
-    t.numbl = a->numbl.data();
-    t.headf = a->headf;
-    t.tailf = a->tailf;
-    t.numbf = a->numbf;
-    t.next = a->next.data();
-    t.prev = a->prev.data();
-    t.father = a->father.data();
-    t.nbor = a->nbor.data();
-    t.son = a->son.data();
-    t.flag1 = a->flag1.data();
-    t.cpu_map = nullptr;  // every cell belongs to rank 1
-    t.xg = a->xg.data();
-    return rhi::dump_snapshot(a->ctx, &t, &in, root);
+    return guard(a ? &a->err : &g_create_error, (a && a->ctx) ? &a->ctx->err : nullptr, [&]() -> int {
+        if (!a || !root) return RH_ERR_ARG;
+        rh_snapshot_info in{};
+        if (info) {
+            in = *info;
+        } else {  // SnapshotInfo's defaults (RamsesWriter.hpp:24-31), no outputs, unit scales 1
+            in.aexp_ini = in.boxlen_ini = in.aexp = in.aexp_old = 1.0;
+            in.units_length = in.units_density = in.units_time = 1.0;
+        }
+        // What Simulation::dump_snapshot takes from the run (Simulation.cpp:683-692).
+        RH_ACHK(sync_times(a));
+        in.t = a->t;
+        in.nstep = in.nstep_coarse = a->nstep;
+        in.iout = iout;
+        in.levelmin = a->levelmin;
+        in.levelmax = a->nlevelmax;
+        in.boxlen = a->boxlen;
+        rh_snapshot_tree t{};
+        t.nx = a->nx;
+        t.ny = a->ny;
+        t.nz = a->nz;
+        t.ncoarse = a->ncoarse;
+        t.ngridmax = a->ngridmax;
+        t.ncell = a->ncell;
+        t.nlevelmax = a->nlevelmax;
+        t.ncpu = 1;
+        t.nboundary = 0;
+        t.gamma = a->gamma;
+        t.headl = a->headl.data();
+        t.taill = a->taill.data();
+        t.numbl = a->numbl.data();
+        t.headf = a->headf;
+        t.tailf = a->tailf;
+        t.numbf = a->numbf;
+        t.next = a->next.data();
+        t.prev = a->prev.data();
+        t.father = a->father.data();
+        t.nbor = a->nbor.data();
+        t.son = a->son.data();
+        t.flag1 = a->flag1.data();
+        t.cpu_map = nullptr;  // every cell belongs to rank 1
+        t.xg = a->xg.data();
+        return rhi::dump_snapshot(a->ctx, &t, &in, root);
+    });
 }
 
 int rh_amr_timers(rh_amr* a, double* ms) {
-    if (!a || !ms) return RH_ERR_ARG;
-    for (int k = 0; k < 5; ++k) ms[k] = a->phase_ms[k];
-    ms[5] = a->ctx->table_ms;
-    return RH_OK;
+    return guard(a ? &a->err : &g_create_error, (a && a->ctx) ? &a->ctx->err : nullptr, [&]() -> int {
+        if (!a || !ms) return RH_ERR_ARG;
+        for (int k = 0; k < 5; ++k) ms[k] = a->phase_ms[k];
+        ms[5] = a->ctx->table_ms;
+        return RH_OK;
+    });
 }
 
 }  // extern "C"

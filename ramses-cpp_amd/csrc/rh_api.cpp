@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "rh_context.h"
+#include "rh_guard.h"
 #include "rh_kernels.h"
 #include "rh_tables.h"
 #include "rhydro.h"
@@ -43,6 +44,7 @@ void free_level(DevLevel& L) {
     L.allocs.clear();
     L.dev = rh::LevelDev{};
     L.epoch = -2;
+    L.ngridmax = L.ncell = -1;
 }
 
 
@@ -84,7 +86,7 @@ int get_level(rh_ctx* c, int ilevel, const rh::LevelDev** out) {
     } clock{c};
     if ((int)c->levels.size() <= ilevel) c->levels.resize(ilevel + 1);
     DevLevel& L = c->levels[ilevel];
-    if (L.epoch != c->mesh.mesh_epoch) {
+    if (L.epoch != c->mesh.mesh_epoch || L.ngridmax != c->mesh.ngridmax || L.ncell != c->mesh.ncell) {
         // The topology index is built only when some level's tables are.
         if (!c->topo_valid || c->topo.epoch != c->mesh.mesh_epoch) {
             std::string e;
@@ -96,6 +98,7 @@ int get_level(rh_ctx* c, int ilevel, const rh::LevelDev** out) {
         rh::LevelHost h;
         std::string e;
         int st = rh::build_level(c->mesh, c->topo, ilevel, h, e);
+        if (st == RH_OK) st = rh::validate_level(h, c->mesh.ngridmax, c->mesh.ncell, e);
         if (st != RH_OK) return fail(c, st, e);
         rh::LevelDev& d = L.dev;
         d.noct = h.noct;
@@ -121,6 +124,8 @@ int get_level(rh_ctx* c, int ilevel, const rh::LevelDev** out) {
         }
         RH_HIP(c, hipStreamSynchronize(c->stream));  // host vectors go out of scope
         L.epoch = c->mesh.mesh_epoch;
+        L.ngridmax = c->mesh.ngridmax;
+        L.ncell = c->mesh.ncell;
     }
     *out = &L.dev;
     return RH_OK;
@@ -312,398 +317,436 @@ using namespace rhi;
 extern "C" {
 
 int rh_create(const rh_params* params, rh_ctx** out) {
-    if (!params || !out) return fail(nullptr, RH_ERR_ARG, "null argument");
-    *out = nullptr;
-    if (params->nvar != 5 || params->nener != 0)
-        return fail(nullptr, RH_ERR_UNSUPPORTED, "only nvar=5 (NDIM=3, nener=0, no passive scalars) is supported");
-    if (params->barotropic_eos) return fail(nullptr, RH_ERR_UNSUPPORTED, "barotropic EOS is not supported");
-    if (params->slope_type < 0 || params->slope_type > 8) return fail(nullptr, RH_ERR_ARG, "slope_type outside 0..8");
-    if (params->riemann < 0 || params->riemann > 4) return fail(nullptr, RH_ERR_ARG, "unknown riemann id");
-    if (params->riemann == RH_RIEMANN_EXACT)
-        return fail(nullptr, RH_ERR_UNSUPPORTED, "the exact (Newton-Raphson) Riemann solver is not on the GPU path");
-    if (params->precision != RH_PREC_FP64 && params->precision != RH_PREC_MIXED)
-        return fail(nullptr, RH_ERR_ARG, "unknown precision");
-    if (!(params->gamma > 1.0)) return fail(nullptr, RH_ERR_ARG, "gamma must exceed 1");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-        return fail(nullptr, RH_ERR_NO_DEVICE, "no HIP device available");
-    if (params->device < 0 || params->device >= ndev) return fail(nullptr, RH_ERR_ARG, "device ordinal out of range");
-    std::unique_ptr<rh_ctx> c(new rh_ctx());
-    c->p = *params;
-    hipError_t e = hipSetDevice(params->device);
-    if (e != hipSuccess) return fail(nullptr, RH_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
-    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return fail(nullptr, RH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-    *out = c.release();
-    return RH_OK;
+    return guard(&g_create_error, nullptr, [&]() -> int {
+        if (!params || !out) return fail(nullptr, RH_ERR_ARG, "null argument");
+        *out = nullptr;
+        if (params->nvar != 5 || params->nener != 0)
+            return fail(nullptr, RH_ERR_UNSUPPORTED, "only nvar=5 (NDIM=3, nener=0, no passive scalars) is supported");
+        if (params->barotropic_eos) return fail(nullptr, RH_ERR_UNSUPPORTED, "barotropic EOS is not supported");
+        if (params->slope_type < 0 || params->slope_type > 8) return fail(nullptr, RH_ERR_ARG, "slope_type outside 0..8");
+        if (params->riemann < 0 || params->riemann > 4) return fail(nullptr, RH_ERR_ARG, "unknown riemann id");
+        if (params->riemann == RH_RIEMANN_EXACT)
+            return fail(nullptr, RH_ERR_UNSUPPORTED, "the exact (Newton-Raphson) Riemann solver is not on the GPU path");
+        if (params->precision != RH_PREC_FP64 && params->precision != RH_PREC_MIXED)
+            return fail(nullptr, RH_ERR_ARG, "unknown precision");
+        if (!(params->gamma > 1.0)) return fail(nullptr, RH_ERR_ARG, "gamma must exceed 1");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+            return fail(nullptr, RH_ERR_NO_DEVICE, "no HIP device available");
+        if (params->device < 0 || params->device >= ndev) return fail(nullptr, RH_ERR_ARG, "device ordinal out of range");
+        std::unique_ptr<rh_ctx> c(new rh_ctx());
+        c->p = *params;
+        hipError_t e = hipSetDevice(params->device);
+        if (e != hipSuccess) return fail(nullptr, RH_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return fail(nullptr, RH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        *out = c.release();
+        return RH_OK;
+    });
 }
 
 int rh_destroy(rh_ctx* c) {
-    if (!c) return RH_OK;
-    (void)hipSetDevice(c->p.device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    unbind(c);
-    for (void* p : {(void*)c->d_part, (void*)c->d_dt, (void*)c->d_u1, (void*)c->d_u2})
-        if (p) (void)hipFree(p);
-    collect_timers(c);
-    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    delete c;
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_OK;
+        (void)hipSetDevice(c->p.device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        unbind(c);
+        for (void* p : {(void*)c->d_part, (void*)c->d_dt, (void*)c->d_u1, (void*)c->d_u2})
+            if (p) (void)hipFree(p);
+        collect_timers(c);
+        for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return RH_OK;
+    });
 }
 
 int rh_bind(rh_ctx* c, const rh_mesh* m, int mode) {
-    if (!c || !m) return RH_ERR_ARG;
-    if (mode != RH_HOST_COHERENT && mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "unknown mode");
-    if (m->nvar != 5) return fail(c, RH_ERR_UNSUPPORTED, "mesh nvar must be 5");
-    if (m->ncoarse != m->nx * m->ny * m->nz || m->ncell != m->ncoarse + 8 * m->ngridmax)
-        return fail(c, RH_ERR_ARG, "ncoarse/ncell inconsistent with nx,ny,nz,ngridmax");
-    if (m->ncpu < 1 || m->myid < 1 || m->myid > m->ncpu + m->nboundary || m->nlevelmax < 1)
-        return fail(c, RH_ERR_ARG, "bad ncpu/myid/nlevelmax");
-    if (!m->father || !m->son || !m->nbor || !m->next || !m->headl || !m->uold || !m->unew)
-        return fail(c, RH_ERR_ARG, "null mesh array");
-    RH_HIP(c, hipSetDevice(c->p.device));
-    if (c->ens) unbind(c);
-    const size_t n = (size_t)m->nvar * m->ncell;
-    bool topo_changed = !c->bound || c->mesh.mesh_epoch != m->mesh_epoch || c->mesh.father != m->father ||
-                        c->mesh.son != m->son || c->mesh.nbor != m->nbor || c->mesh.next != m->next ||
-                        c->mesh.headl != m->headl || c->mesh.ncell != m->ncell;
-    if (n > c->state_cap) {
-        if (c->d_uold) (void)hipFree(c->d_uold);
-        if (c->d_unew) (void)hipFree(c->d_unew);
-        c->d_uold = c->d_unew = nullptr;
-        c->state_cap = 0;
-        RH_HIP(c, hipMalloc(&c->d_uold, n * sizeof(double)));
-        RH_HIP(c, hipMalloc(&c->d_unew, n * sizeof(double)));
-        c->state_cap = n;
-    }
-    c->mesh = *m;
-    c->mode = mode;
-    c->bound = true;
-    if (topo_changed) {
-        c->topo_valid = false;
-        for (auto& L : c->levels) free_level(L);
-    }
-    int st = upload_state(c, RH_UOLD | RH_UNEW);
-    if (st) return st;
-    if (m->f) {
-        size_t nf = (size_t)3 * m->ncell;
-        if (nf > c->f_cap) {
-            if (c->d_f) (void)hipFree(c->d_f);
-            c->d_f = nullptr;
-            RH_HIP(c, hipMalloc(&c->d_f, nf * sizeof(double)));
-            c->f_cap = nf;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c || !m) return RH_ERR_ARG;
+        if (mode != RH_HOST_COHERENT && mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "unknown mode");
+        if (m->nvar != 5) return fail(c, RH_ERR_UNSUPPORTED, "mesh nvar must be 5");
+        if (m->ncoarse != m->nx * m->ny * m->nz || m->ncell != m->ncoarse + 8 * m->ngridmax)
+            return fail(c, RH_ERR_ARG, "ncoarse/ncell inconsistent with nx,ny,nz,ngridmax");
+        if (m->ncpu < 1 || m->myid < 1 || m->myid > m->ncpu + m->nboundary || m->nlevelmax < 1)
+            return fail(c, RH_ERR_ARG, "bad ncpu/myid/nlevelmax");
+        if (!m->father || !m->son || !m->nbor || !m->next || !m->headl || !m->uold || !m->unew)
+            return fail(c, RH_ERR_ARG, "null mesh array");
+        RH_HIP(c, hipSetDevice(c->p.device));
+        if (c->ens) unbind(c);
+        const size_t n = (size_t)m->nvar * m->ncell;
+        bool topo_changed = !c->bound || c->mesh.mesh_epoch != m->mesh_epoch || c->mesh.father != m->father ||
+                            c->mesh.son != m->son || c->mesh.nbor != m->nbor || c->mesh.next != m->next ||
+                            c->mesh.headl != m->headl || c->mesh.ncell != m->ncell;
+        if (n > c->state_cap) {
+            if (c->d_uold) (void)hipFree(c->d_uold);
+            if (c->d_unew) (void)hipFree(c->d_unew);
+            c->d_uold = c->d_unew = nullptr;
+            c->state_cap = 0;
+            RH_HIP(c, hipMalloc(&c->d_uold, n * sizeof(double)));
+            RH_HIP(c, hipMalloc(&c->d_unew, n * sizeof(double)));
+            c->state_cap = n;
         }
-        RH_HIP(c, hipMemcpyAsync(c->d_f, m->f, nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    } else if (c->d_f) {
-        (void)hipFree(c->d_f);
-        c->d_f = nullptr;
-        c->f_cap = 0;
-    }
-    RH_HIP(c, hipStreamSynchronize(c->stream));
-    return RH_OK;
+        c->mesh = *m;
+        c->mode = mode;
+        c->bound = true;
+        if (topo_changed) {
+            c->topo_valid = false;
+            for (auto& L : c->levels) free_level(L);
+        }
+        int st = upload_state(c, RH_UOLD | RH_UNEW);
+        if (st) return st;
+        if (m->f) {
+            size_t nf = (size_t)3 * m->ncell;
+            if (nf > c->f_cap) {
+                if (c->d_f) (void)hipFree(c->d_f);
+                c->d_f = nullptr;
+                RH_HIP(c, hipMalloc(&c->d_f, nf * sizeof(double)));
+                c->f_cap = nf;
+            }
+            RH_HIP(c, hipMemcpyAsync(c->d_f, m->f, nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        } else if (c->d_f) {
+            (void)hipFree(c->d_f);
+            c->d_f = nullptr;
+            c->f_cap = 0;
+        }
+        RH_HIP(c, hipStreamSynchronize(c->stream));
+        return RH_OK;
+    });
 }
 
 int rh_set_unew(rh_ctx* c, int ilevel) {
-    int st = check_bound(c);
-    if (st) return st;
-    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
-    if ((st = pre_call(c))) return st;
-    const rh::LevelDev* lv;
-    if ((st = get_level(c, ilevel, &lv))) return st;
-    Timed tm(c, TK_SET_UNEW);
-    RH_HIP(c, rh::launch_set_unew(mesh_dev(c), *lv, c->stream));
-    tm.done();
-    return post_call(c, RH_UNEW);
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+        if ((st = pre_call(c))) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        Timed tm(c, TK_SET_UNEW);
+        RH_HIP(c, rh::launch_set_unew(mesh_dev(c), *lv, c->stream));
+        tm.done();
+        return post_call(c, RH_UNEW);
+    });
 }
 
 int rh_set_uold(rh_ctx* c, int ilevel) {
-    int st = check_bound(c);
-    if (st) return st;
-    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
-    if ((st = pre_call(c))) return st;
-    const rh::LevelDev* lv;
-    if ((st = get_level(c, ilevel, &lv))) return st;
-    Timed tm(c, TK_SET_UOLD);
-    RH_HIP(c, rh::launch_set_uold(mesh_dev(c), *lv, c->stream));
-    tm.done();
-    return post_call(c, RH_UOLD);
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+        if ((st = pre_call(c))) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        Timed tm(c, TK_SET_UOLD);
+        RH_HIP(c, rh::launch_set_uold(mesh_dev(c), *lv, c->stream));
+        tm.done();
+        return post_call(c, RH_UOLD);
+    });
 }
 
 int rh_add_gravity_source_terms(rh_ctx* c, int ilevel, double dt) {
-    int st = check_bound(c);
-    if (st) return st;
-    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
-    if ((st = pre_call(c))) return st;
-    const rh::LevelDev* lv;
-    if ((st = get_level(c, ilevel, &lv))) return st;
-    RH_HIP(c, rh::launch_gravity_source(mesh_dev(c), *lv, dt, c->stream));
-    return post_call(c, RH_UNEW);
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+        if ((st = pre_call(c))) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        RH_HIP(c, rh::launch_gravity_source(mesh_dev(c), *lv, dt, c->stream));
+        return post_call(c, RH_UNEW);
+    });
 }
 
 int rh_synchro_hydro_fine(rh_ctx* c, int ilevel, double dt) {
-    int st = check_bound(c);
-    if (st) return st;
-    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
-    if ((st = pre_call(c))) return st;
-    const rh::LevelDev* lv;
-    if ((st = get_level(c, ilevel, &lv))) return st;
-    RH_HIP(c, rh::launch_synchro(mesh_dev(c), *lv, dt, c->p.gamma, c->stream));
-    return post_call(c, RH_UOLD);
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+        if ((st = pre_call(c))) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        RH_HIP(c, rh::launch_synchro(mesh_dev(c), *lv, dt, c->p.gamma, c->stream));
+        return post_call(c, RH_UOLD);
+    });
 }
 
 int rh_godunov_fine(rh_ctx* c, int ilevel, double dt, double dx) {
-    int st = check_bound(c);
-    if (st) return st;
-    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
-    if ((st = pre_call(c))) return st;
-    const rh::LevelDev* lv;
-    if ((st = get_level(c, ilevel, &lv))) return st;
-    if ((st = ensure_dt(c, 1))) return st;
-    RH_HIP(c, hipMemcpyAsync(c->d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if ((st = do_godunov(c, lv, c->d_dt, dx))) return st;
-    RH_HIP(c, hipStreamSynchronize(c->stream));  // dt staging value is on the stack
-    return post_call(c, RH_UNEW);
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+        if ((st = pre_call(c))) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        if ((st = ensure_dt(c, 1))) return st;
+        RH_HIP(c, hipMemcpyAsync(c->d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        if ((st = do_godunov(c, lv, c->d_dt, dx))) return st;
+        RH_HIP(c, hipStreamSynchronize(c->stream));  // dt staging value is on the stack
+        return post_call(c, RH_UNEW);
+    });
 }
 
 int rh_courant(rh_ctx* c, int ilevel, double dx, double gamma, double courant_factor, double* dt_out) {
-    int st = check_bound(c);
-    if (st) return st;
-    if (!dt_out) return fail(c, RH_ERR_ARG, "null dt_out");
-    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
-    if ((st = pre_call(c))) return st;
-    const rh::LevelDev* lv;
-    if ((st = get_level(c, ilevel, &lv))) return st;
-    if ((st = ensure_dt(c, 1))) return st;
-    if ((st = do_courant(c, lv, dx, gamma, courant_factor, c->d_dt))) return st;
-    RH_HIP(c, hipMemcpyAsync(dt_out, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    RH_HIP(c, hipStreamSynchronize(c->stream));
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (!dt_out) return fail(c, RH_ERR_ARG, "null dt_out");
+        if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+        if ((st = pre_call(c))) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        if ((st = ensure_dt(c, 1))) return st;
+        if ((st = do_courant(c, lv, dx, gamma, courant_factor, c->d_dt))) return st;
+        RH_HIP(c, hipMemcpyAsync(dt_out, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        RH_HIP(c, hipStreamSynchronize(c->stream));
+        return RH_OK;
+    });
 }
 
 int rh_interpol_hydro(rh_ctx* c, int n_octs, const double* u1, double* u2) {
-    if (!c) return RH_ERR_ARG;
-    if (n_octs < 0 || (n_octs > 0 && (!u1 || !u2))) return fail(c, RH_ERR_ARG, "bad interpol arguments");
-    if (n_octs == 0) return RH_OK;
-    RH_HIP(c, hipSetDevice(c->p.device));
-    if (n_octs > c->interpol_cap) {
-        if (c->d_u1) (void)hipFree(c->d_u1);
-        if (c->d_u2) (void)hipFree(c->d_u2);
-        c->d_u1 = c->d_u2 = nullptr;
-        c->interpol_cap = 0;
-        RH_HIP(c, hipMalloc(&c->d_u1, (size_t)n_octs * 7 * 64 * sizeof(double)));
-        RH_HIP(c, hipMalloc(&c->d_u2, (size_t)n_octs * 8 * 64 * sizeof(double)));
-        c->interpol_cap = n_octs;
-    }
-    RH_HIP(c, hipMemcpyAsync(c->d_u1, u1, (size_t)n_octs * 7 * 64 * sizeof(double), hipMemcpyHostToDevice,
-                             c->stream));
-    // Rows beyond nvar are returned unchanged from what the caller passed.
-    RH_HIP(c, hipMemcpyAsync(c->d_u2, u2, (size_t)n_octs * 8 * 64 * sizeof(double), hipMemcpyHostToDevice,
-                             c->stream));
-    RH_HIP(c, rh::launch_interpol(n_octs, c->d_u1, c->d_u2, gas(c, c->p.gamma), c->p.interpol_var, c->stream));
-    RH_HIP(c, hipMemcpyAsync(u2, c->d_u2, (size_t)n_octs * 8 * 64 * sizeof(double), hipMemcpyDeviceToHost,
-                             c->stream));
-    RH_HIP(c, hipStreamSynchronize(c->stream));
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        if (n_octs < 0 || (n_octs > 0 && (!u1 || !u2))) return fail(c, RH_ERR_ARG, "bad interpol arguments");
+        if (n_octs == 0) return RH_OK;
+        RH_HIP(c, hipSetDevice(c->p.device));
+        if (n_octs > c->interpol_cap) {
+            if (c->d_u1) (void)hipFree(c->d_u1);
+            if (c->d_u2) (void)hipFree(c->d_u2);
+            c->d_u1 = c->d_u2 = nullptr;
+            c->interpol_cap = 0;
+            RH_HIP(c, hipMalloc(&c->d_u1, (size_t)n_octs * 7 * 64 * sizeof(double)));
+            RH_HIP(c, hipMalloc(&c->d_u2, (size_t)n_octs * 8 * 64 * sizeof(double)));
+            c->interpol_cap = n_octs;
+        }
+        RH_HIP(c, hipMemcpyAsync(c->d_u1, u1, (size_t)n_octs * 7 * 64 * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        // Rows beyond nvar are returned unchanged from what the caller passed.
+        RH_HIP(c, hipMemcpyAsync(c->d_u2, u2, (size_t)n_octs * 8 * 64 * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        RH_HIP(c, rh::launch_interpol(n_octs, c->d_u1, c->d_u2, gas(c, c->p.gamma), c->p.interpol_var, c->stream));
+        RH_HIP(c, hipMemcpyAsync(u2, c->d_u2, (size_t)n_octs * 8 * 64 * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        RH_HIP(c, hipStreamSynchronize(c->stream));
+        return RH_OK;
+    });
 }
 
 int rh_sync(rh_ctx* c, int direction, int what) {
-    if (!c) return RH_ERR_ARG;
-    if (!c->bound) return fail(c, RH_ERR_ARG, "no mesh bound (call rh_bind first)");
-    int st;
-    if (direction == RH_SYNC_TO_HOST) return download_state(c, what);
-    if (direction == RH_SYNC_TO_DEVICE) {
-        if ((st = upload_state(c, what))) return st;
-        RH_HIP(c, hipStreamSynchronize(c->stream));
-        return RH_OK;
-    }
-    return fail(c, RH_ERR_ARG, "unknown sync direction");
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        if (!c->bound) return fail(c, RH_ERR_ARG, "no mesh bound (call rh_bind first)");
+        int st;
+        if (direction == RH_SYNC_TO_HOST) return download_state(c, what);
+        if (direction == RH_SYNC_TO_DEVICE) {
+            if ((st = upload_state(c, what))) return st;
+            RH_HIP(c, hipStreamSynchronize(c->stream));
+            return RH_OK;
+        }
+        return fail(c, RH_ERR_ARG, "unknown sync direction");
+    });
 }
 
 const char* rh_last_error(const rh_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
 
 int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int nsteps, double* dt_out) {
-    int st = check_bound(c);
-    if (st) return st;
-    if (nsteps < 0) return fail(c, RH_ERR_ARG, "nsteps < 0");
-    if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
-    if ((st = pre_call(c))) return st;
-    const rh::LevelDev* lv;
-    if ((st = get_level(c, ilevel, &lv))) return st;
-    if ((st = ensure_dt(c, nsteps > 0 ? nsteps : 1))) return st;
-    const rh::MeshDev m = mesh_dev(c);
-    // A level of leaf cells whose neighbours are all at the same level (a
-    // uniform periodic level) steps with uold and unew exchanged instead of
-    // copied: the Godunov launch of step k reads the state from one buffer,
-    // writes the new state into the other (set_unew's copy is the buffer it
-    // reads, set_uold's copy is the exchange) and computes the next step's
-    // Courant candidates from the values it writes.  At the end the level's
-    // cells of both buffers hold the final state, as after set_uold.
-    const bool fused = c->variant != 0 && lv->ntgt == 0 && lv->nrec == 0 && lv->nleaf == 8LL * lv->noct &&
-                       lv->noct > 0;
-    if (fused) {
-        const int nb = rh::godunov_blocks(*lv, c->variant);
-        if ((st = ensure_part(c, nb + 1))) return st;
-        rh::MeshDev mk = m;
-        for (int k = 0; k < nsteps; ++k) {
-            if (k == 0 && (st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt))) return st;
-            const bool next = k + 1 < nsteps;
-            rh::FusedCourant fc;
-            fc.pre_from_uold = true;
-            fc.part = next ? c->d_part : nullptr;
-            fc.courant_factor = courant_factor;
-            {
-                Timed tm(c, TK_GODUNOV);
-                hipError_t e = rh::launch_godunov(mk, *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision,
-                                                  c->variant, c->d_dt + k, dx, exact_inverse(dx), c->stream, fc);
-                if (e == hipErrorNotSupported)
-                    return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
-                if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (nsteps < 0) return fail(c, RH_ERR_ARG, "nsteps < 0");
+        if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+        if ((st = pre_call(c))) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        if ((st = ensure_dt(c, nsteps > 0 ? nsteps : 1))) return st;
+        const rh::MeshDev m = mesh_dev(c);
+        // A level of leaf cells whose neighbours are all at the same level (a
+        // uniform periodic level) steps with uold and unew exchanged instead of
+        // copied: the Godunov launch of step k reads the state from one buffer,
+        // writes the new state into the other (set_unew's copy is the buffer it
+        // reads, set_uold's copy is the exchange) and computes the next step's
+        // Courant candidates from the values it writes.  At the end the level's
+        // cells of both buffers hold the final state, as after set_uold.
+        const bool fused = c->variant != 0 && lv->ntgt == 0 && lv->nrec == 0 && lv->nleaf == 8LL * lv->noct &&
+                           lv->noct > 0;
+        if (fused) {
+            const int nb = rh::godunov_blocks(*lv, c->variant);
+            if ((st = ensure_part(c, nb + 1))) return st;
+            rh::MeshDev mk = m;
+            for (int k = 0; k < nsteps; ++k) {
+                if (k == 0 && (st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt))) return st;
+                const bool next = k + 1 < nsteps;
+                rh::FusedCourant fc;
+                fc.pre_from_uold = true;
+                fc.part = next ? c->d_part : nullptr;
+                fc.courant_factor = courant_factor;
+                {
+                    Timed tm(c, TK_GODUNOV);
+                    hipError_t e = rh::launch_godunov(mk, *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision,
+                                                      c->variant, c->d_dt + k, dx, exact_inverse(dx), c->stream, fc);
+                    if (e == hipErrorNotSupported)
+                        return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
+                    if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
+                    tm.done();
+                }
+                if (next) {
+                    Timed tm(c, TK_COURANT);
+                    RH_HIP(c, rh::launch_courant_final(c->d_part, nb, c->p.boxlen / 1e-10, c->d_dt + k + 1, c->stream));
+                    tm.done();
+                }
+                std::swap(mk.uold, mk.unew);
+            }
+            if (nsteps > 0) {
+                // mk.uold holds the final state; copy the level's cells into the
+                // other buffer (set_unew's copy, uold -> unew).
+                Timed tm(c, TK_SET_UOLD);
+                RH_HIP(c, rh::launch_set_unew(mk, *lv, c->stream));
                 tm.done();
             }
-            if (next) {
-                Timed tm(c, TK_COURANT);
-                RH_HIP(c, rh::launch_courant_final(c->d_part, nb, c->p.boxlen / 1e-10, c->d_dt + k + 1, c->stream));
+        }
+        // Otherwise step k is courant -> set_unew -> godunov_fine -> set_uold.
+        // Between two steps, set_uold of the first and courant + set_unew of the
+        // second run as one fused pass (k_step_join), with the same state and
+        // time step.
+        for (int k = 0; k < (fused ? 0 : nsteps); ++k) {
+            if (k == 0) {
+                if ((st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt))) return st;
+                Timed tm(c, TK_SET_UNEW);
+                RH_HIP(c, rh::launch_set_unew(m, *lv, c->stream));
                 tm.done();
             }
-            std::swap(mk.uold, mk.unew);
+            if ((st = do_godunov(c, lv, c->d_dt + k, dx))) return st;
+            if (k + 1 < nsteps) {
+                if ((st = ensure_part(c, rh::courant_blocks(*lv) + 1))) return st;
+                Timed tm(c, TK_STEP_JOIN);
+                RH_HIP(c, rh::launch_step_join(m, *lv, dx, c->p.gamma, courant_factor, c->p.boxlen / 1e-10, c->d_part,
+                                               c->npart, c->d_dt + k + 1, c->stream));
+                tm.done();
+            } else {
+                Timed tm(c, TK_SET_UOLD);
+                RH_HIP(c, rh::launch_set_uold(m, *lv, c->stream));
+                tm.done();
+            }
         }
-        if (nsteps > 0) {
-            // mk.uold holds the final state; copy the level's cells into the
-            // other buffer (set_unew's copy, uold -> unew).
-            Timed tm(c, TK_SET_UOLD);
-            RH_HIP(c, rh::launch_set_unew(mk, *lv, c->stream));
-            tm.done();
-        }
-    }
-    // Otherwise step k is courant -> set_unew -> godunov_fine -> set_uold.
-    // Between two steps, set_uold of the first and courant + set_unew of the
-    // second run as one fused pass (k_step_join), with the same state and
-    // time step.
-    for (int k = 0; k < (fused ? 0 : nsteps); ++k) {
-        if (k == 0) {
-            if ((st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt))) return st;
-            Timed tm(c, TK_SET_UNEW);
-            RH_HIP(c, rh::launch_set_unew(m, *lv, c->stream));
-            tm.done();
-        }
-        if ((st = do_godunov(c, lv, c->d_dt + k, dx))) return st;
-        if (k + 1 < nsteps) {
-            if ((st = ensure_part(c, rh::courant_blocks(*lv) + 1))) return st;
-            Timed tm(c, TK_STEP_JOIN);
-            RH_HIP(c, rh::launch_step_join(m, *lv, dx, c->p.gamma, courant_factor, c->p.boxlen / 1e-10, c->d_part,
-                                           c->npart, c->d_dt + k + 1, c->stream));
-            tm.done();
-        } else {
-            Timed tm(c, TK_SET_UOLD);
-            RH_HIP(c, rh::launch_set_uold(m, *lv, c->stream));
-            tm.done();
-        }
-    }
-    if (dt_out && nsteps > 0)
-        RH_HIP(c, hipMemcpyAsync(dt_out, c->d_dt, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
-    RH_HIP(c, hipStreamSynchronize(c->stream));
-    return post_call(c, RH_UOLD | RH_UNEW);
+        if (dt_out && nsteps > 0)
+            RH_HIP(c, hipMemcpyAsync(dt_out, c->d_dt, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+        RH_HIP(c, hipStreamSynchronize(c->stream));
+        return post_call(c, RH_UOLD | RH_UNEW);
+    });
 }
 
 int rh_set_kernel_variant(rh_ctx* c, int variant) {
-    if (!c) return RH_ERR_ARG;
-    if (variant < 0 || variant >= RH_KERNEL_VARIANTS) return fail(c, RH_ERR_ARG, "unknown kernel variant");
-    c->variant = variant;
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        if (variant < 0 || variant >= RH_KERNEL_VARIANTS) return fail(c, RH_ERR_ARG, "unknown kernel variant");
+        c->variant = variant;
+        return RH_OK;
+    });
 }
 
 int rh_autotune(rh_ctx* c, int ilevel, double dt, double dx, int reps, int* best_variant, double* ms_per_variant) {
-    int st = check_bound(c);
-    if (st) return st;
-    if (reps < 1) return fail(c, RH_ERR_ARG, "reps < 1");
-    if ((st = pre_call(c))) return st;
-    const rh::LevelDev* lv;
-    if ((st = get_level(c, ilevel, &lv))) return st;
-    if ((st = ensure_dt(c, 1))) return st;
-    RH_HIP(c, hipMemcpyAsync(c->d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    // Every variant computes the same unew; a saved copy is restored after
-    // each trial so the bound state is left as it was.
-    const size_t bytes = state_bytes(c);
-    double* saved = nullptr;
-    RH_HIP(c, hipMalloc(&saved, bytes));
-    RH_HIP(c, hipMemcpyAsync(saved, c->d_unew, bytes, hipMemcpyDeviceToDevice, c->stream));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    const int keep = c->variant;
-    int best = keep;
-    float best_ms = 0.f;
-    int rc = RH_OK;
-    static_assert(RH_KERNEL_VARIANTS == rh::KERNEL_VARIANTS, "variant count of the C-ABI and the kernels");
-    for (int v = 0; v < RH_KERNEL_VARIANTS && rc == RH_OK; ++v) {
-        c->variant = v;
-        float total = 0.f;
-        for (int r = 0; r <= reps && rc == RH_OK; ++r) {  // first launch untimed
-            (void)hipEventRecord(e0, c->stream);
-            rc = do_godunov(c, lv, c->d_dt, dx);
-            (void)hipEventRecord(e1, c->stream);
-            (void)hipEventSynchronize(e1);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (r > 0) total += ms;
-            if (hipMemcpyAsync(c->d_unew, saved, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
-                rc = fail(c, RH_ERR_HIP, "restore unew");
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (reps < 1) return fail(c, RH_ERR_ARG, "reps < 1");
+        if ((st = pre_call(c))) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        if ((st = ensure_dt(c, 1))) return st;
+        RH_HIP(c, hipMemcpyAsync(c->d_dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        // Every variant computes the same unew; a saved copy is restored after
+        // each trial so the bound state is left as it was.
+        const size_t bytes = state_bytes(c);
+        double* saved = nullptr;
+        RH_HIP(c, hipMalloc(&saved, bytes));
+        RH_HIP(c, hipMemcpyAsync(saved, c->d_unew, bytes, hipMemcpyDeviceToDevice, c->stream));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        const int keep = c->variant;
+        int best = keep;
+        float best_ms = 0.f;
+        int rc = RH_OK;
+        static_assert(RH_KERNEL_VARIANTS == rh::KERNEL_VARIANTS, "variant count of the C-ABI and the kernels");
+        for (int v = 0; v < RH_KERNEL_VARIANTS && rc == RH_OK; ++v) {
+            c->variant = v;
+            float total = 0.f;
+            for (int r = 0; r <= reps && rc == RH_OK; ++r) {  // first launch untimed
+                (void)hipEventRecord(e0, c->stream);
+                rc = do_godunov(c, lv, c->d_dt, dx);
+                (void)hipEventRecord(e1, c->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (r > 0) total += ms;
+                if (hipMemcpyAsync(c->d_unew, saved, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                    rc = fail(c, RH_ERR_HIP, "restore unew");
+            }
+            const float avg = total / reps;
+            if (ms_per_variant) ms_per_variant[v] = avg;
+            if (v == 0 || avg < best_ms) {
+                best_ms = avg;
+                best = v;
+            }
         }
-        const float avg = total / reps;
-        if (ms_per_variant) ms_per_variant[v] = avg;
-        if (v == 0 || avg < best_ms) {
-            best_ms = avg;
-            best = v;
-        }
-    }
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(saved);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    c->variant = rc == RH_OK ? best : keep;
-    if (best_variant) *best_variant = c->variant;
-    return rc;
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(saved);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->variant = rc == RH_OK ? best : keep;
+        if (best_variant) *best_variant = c->variant;
+        return rc;
+    });
 }
 
 int rh_get_kernel_variant(rh_ctx* c, int* variant) {
-    if (!c || !variant) return RH_ERR_ARG;
-    *variant = c->variant;
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c || !variant) return RH_ERR_ARG;
+        *variant = c->variant;
+        return RH_OK;
+    });
 }
 
 int rh_timers_enable(rh_ctx* c, int enable) {
-    if (!c) return RH_ERR_ARG;
-    c->timers = enable != 0;
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        c->timers = enable != 0;
+        return RH_OK;
+    });
 }
 
 int rh_timers_read(rh_ctx* c, int kernel, double* ms_total, long long* count) {
-    if (!c || kernel < 0 || kernel >= TK_N) return RH_ERR_ARG;
-    collect_timers(c);
-    if (ms_total) *ms_total = c->t_ms[kernel];
-    if (count) *count = c->t_n[kernel];
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c || kernel < 0 || kernel >= TK_N) return RH_ERR_ARG;
+        collect_timers(c);
+        if (ms_total) *ms_total = c->t_ms[kernel];
+        if (count) *count = c->t_n[kernel];
+        return RH_OK;
+    });
 }
 
 int rh_timers_reset(rh_ctx* c) {
-    if (!c) return RH_ERR_ARG;
-    collect_timers(c);
-    for (int k = 0; k < TK_N; ++k) {
-        c->t_ms[k] = 0;
-        c->t_n[k] = 0;
-    }
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        collect_timers(c);
+        for (int k = 0; k < TK_N; ++k) {
+            c->t_ms[k] = 0;
+            c->t_n[k] = 0;
+        }
+        return RH_OK;
+    });
 }
 
 int rh_level_leaf_cells(rh_ctx* c, int ilevel, long long* nleaf) {
-    int st = check_bound(c);
-    if (st) return st;
-    const rh::LevelDev* lv;
-    if ((st = get_level(c, ilevel, &lv))) return st;
-    if (nleaf) *nleaf = lv->nleaf;
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        if (nleaf) *nleaf = lv->nleaf;
+        return RH_OK;
+    });
 }
 
 }  // extern "C"

@@ -16,6 +16,10 @@ namespace rhi {
 
 struct DevLevel {
     long long epoch = -2;
+    // Sizes of the mesh the tables were built and range-checked against; a
+    // binding of other sizes rebuilds them even under the same epoch.
+    int ngridmax = -1;
+    int ncell = -1;
     rh::LevelDev dev;
     std::vector<void*> allocs;
 };

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "rhydro.h"
+#include "rh_guard.h"
 #include "rhydro_driver.h"
 
 namespace {
@@ -198,85 +199,91 @@ int rh_init_regions(const rh_mesh* mp, const double* xg, double boxlen, double g
                     const double* length_x, const double* length_y, const double* length_z, const double* d_region,
                     const double* u_region, const double* v_region, const double* w_region,
                     const double* p_region) {
-    if (!mp || !xg || nreg < 0) return RH_ERR_ARG;
-    const rh_mesh& m = *mp;
-    fill_cells(m, xg, boxlen, [&](int idc, double x, double y, double z) {
-        for (int ir = 0; ir < nreg; ++ir) {
-            bool in = false;
-            if (type[ir] == 0) {
-                in = std::fabs(x - x_center[ir]) < 0.5 * length_x[ir] &&
-                     std::fabs(y - y_center[ir]) < 0.5 * length_y[ir] &&
-                     std::fabs(z - z_center[ir]) < 0.5 * length_z[ir];
-            } else {
-                double r2 = std::pow(x - x_center[ir], 2) + std::pow(y - y_center[ir], 2) +
-                            std::pow(z - z_center[ir], 2);
-                in = r2 < std::pow(length_x[ir], 2);
+    return rhi::guard(&rhi::g_create_error, nullptr, [&]() -> int {
+        if (!mp || !xg || nreg < 0) return RH_ERR_ARG;
+        const rh_mesh& m = *mp;
+        fill_cells(m, xg, boxlen, [&](int idc, double x, double y, double z) {
+            for (int ir = 0; ir < nreg; ++ir) {
+                bool in = false;
+                if (type[ir] == 0) {
+                    in = std::fabs(x - x_center[ir]) < 0.5 * length_x[ir] &&
+                         std::fabs(y - y_center[ir]) < 0.5 * length_y[ir] &&
+                         std::fabs(z - z_center[ir]) < 0.5 * length_z[ir];
+                } else {
+                    double r2 = std::pow(x - x_center[ir], 2) + std::pow(y - y_center[ir], 2) +
+                                std::pow(z - z_center[ir], 2);
+                    in = r2 < std::pow(length_x[ir], 2);
+                }
+                if (!in) continue;
+                set_prim(m, idc, gamma, d_region[ir], u_region[ir], v_region[ir], w_region[ir], p_region[ir]);
             }
-            if (!in) continue;
-            set_prim(m, idc, gamma, d_region[ir], u_region[ir], v_region[ir], w_region[ir], p_region[ir]);
-        }
+        });
+        return RH_OK;
     });
-    return RH_OK;
 }
 
 int rh_init_shock_tube(const rh_mesh* mp, const double* xg, double boxlen, double gamma, const double* normal,
                        const double* point, const double* left, const double* right) {
-    if (!mp || !xg || !normal || !point || !left || !right) return RH_ERR_ARG;
-    const rh_mesh& m = *mp;
-    fill_cells(m, xg, boxlen, [&](int idc, double x, double y, double z) {
-        const double s = (x - point[0]) * normal[0] + (y - point[1]) * normal[1] + (z - point[2]) * normal[2];
-        const double* q = s < 0.0 ? left : right;
-        set_prim(m, idc, gamma, q[0], q[1], q[2], q[3], q[4]);
+    return rhi::guard(&rhi::g_create_error, nullptr, [&]() -> int {
+        if (!mp || !xg || !normal || !point || !left || !right) return RH_ERR_ARG;
+        const rh_mesh& m = *mp;
+        fill_cells(m, xg, boxlen, [&](int idc, double x, double y, double z) {
+            const double s = (x - point[0]) * normal[0] + (y - point[1]) * normal[1] + (z - point[2]) * normal[2];
+            const double* q = s < 0.0 ? left : right;
+            set_prim(m, idc, gamma, q[0], q[1], q[2], q[3], q[4]);
+        });
+        return RH_OK;
     });
-    return RH_OK;
 }
 
 int rh_init_lognormal(const rh_mesh* mp, int level, unsigned long long seed, double gamma, double sigma,
                       double vel_amp) {
-    if (!mp) return RH_ERR_ARG;
-    const rh_mesh& m = *mp;
-    auto U = [&](int icell, int iv) -> double& { return m.uold[(size_t)(iv - 1) * m.ncell + icell - 1]; };
-    for (int i = 1; i <= m.ncell; ++i) {
-        U(i, 1) = 1.0;
-        U(i, 2) = 0.0;
-        U(i, 3) = 0.0;
-        U(i, 4) = 0.0;
-        U(i, 5) = 1.0 / (gamma - 1.0);
-    }
-    std::mt19937_64 rng(seed);
-    bool have = false;
-    double spare = 0.0;
-    const double two_pi = 6.283185307179586;
-    auto normal = [&]() {
-        if (have) {
-            have = false;
-            return spare;
+    return rhi::guard(&rhi::g_create_error, nullptr, [&]() -> int {
+        if (!mp) return RH_ERR_ARG;
+        const rh_mesh& m = *mp;
+        auto U = [&](int icell, int iv) -> double& { return m.uold[(size_t)(iv - 1) * m.ncell + icell - 1]; };
+        for (int i = 1; i <= m.ncell; ++i) {
+            U(i, 1) = 1.0;
+            U(i, 2) = 0.0;
+            U(i, 3) = 0.0;
+            U(i, 4) = 0.0;
+            U(i, 5) = 1.0 / (gamma - 1.0);
         }
-        double u1 = ((double)(rng() >> 11) + 1.0) * 0x1.0p-53;
-        double u2 = ((double)(rng() >> 11) + 1.0) * 0x1.0p-53;
-        double r = std::sqrt(-2.0 * std::log(u1));
-        spare = r * std::sin(two_pi * u2);
-        have = true;
-        return r * std::cos(two_pi * u2);
-    };
-    for (int ig = headl_of(m, level); ig > 0; ig = m.next[ig - 1]) {
-        for (int ic0 = 0; ic0 < 8; ++ic0) {
-            int c = cell_of(m.ncoarse, m.ngridmax, ig, ic0);
-            if (m.son[c - 1] != 0) continue;
-            double rho = std::exp(sigma * normal());
-            double p = std::exp(sigma * normal());
-            double cs = std::sqrt(gamma * p / rho);
-            double v[3];
-            for (int d = 0; d < 3; ++d) v[d] = vel_amp * cs * normal();
-            U(c, 1) = rho;
-            U(c, 2) = rho * v[0];
-            U(c, 3) = rho * v[1];
-            U(c, 4) = rho * v[2];
-            U(c, 5) = p / (gamma - 1.0) + 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        std::mt19937_64 rng(seed);
+        bool have = false;
+        double spare = 0.0;
+        const double two_pi = 6.283185307179586;
+        auto normal = [&]() {
+            if (have) {
+                have = false;
+                return spare;
+            }
+            double u1 = ((double)(rng() >> 11) + 1.0) * 0x1.0p-53;
+            double u2 = ((double)(rng() >> 11) + 1.0) * 0x1.0p-53;
+            double r = std::sqrt(-2.0 * std::log(u1));
+            spare = r * std::sin(two_pi * u2);
+            have = true;
+            return r * std::cos(two_pi * u2);
+        };
+        for (int ig = headl_of(m, level); ig > 0; ig = m.next[ig - 1]) {
+            for (int ic0 = 0; ic0 < 8; ++ic0) {
+                int c = cell_of(m.ncoarse, m.ngridmax, ig, ic0);
+                if (m.son[c - 1] != 0) continue;
+                double rho = std::exp(sigma * normal());
+                double p = std::exp(sigma * normal());
+                double cs = std::sqrt(gamma * p / rho);
+                double v[3];
+                for (int d = 0; d < 3; ++d) v[d] = vel_amp * cs * normal();
+                U(c, 1) = rho;
+                U(c, 2) = rho * v[0];
+                U(c, 3) = rho * v[1];
+                U(c, 4) = rho * v[2];
+                U(c, 5) = p / (gamma - 1.0) + 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            }
         }
-    }
-    for (size_t i = 0; i < (size_t)m.nvar * m.ncell; ++i) m.unew[i] = m.uold[i];
-    return RH_OK;
+        for (size_t i = 0; i < (size_t)m.nvar * m.ncell; ++i) m.unew[i] = m.uold[i];
+        return RH_OK;
+    });
 }
 
 }  // extern "C"

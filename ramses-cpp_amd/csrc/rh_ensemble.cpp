@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "rh_context.h"
+#include "rh_guard.h"
 #include "rh_kernels.h"
 #include "rh_tables.h"
 #include "rhydro.h"
@@ -139,6 +140,13 @@ int build_tables(rh_ctx* c, const int* levels) {
     }
     all.tgt_off.push_back((int)all.tgt_slot.size());
     all.nslots = S;
+    {
+        std::string e;
+        const int st = rh::validate_level(all, T.ngridmax, T.ncell, e);
+        if (st != RH_OK) return fail(c, st, "ensemble: " + e);
+        for (size_t p = 0; p < obox.size(); ++p)
+            if (obox[p] < 0 || obox[p] >= nbox) return fail(c, RH_ERR_TOPOLOGY, "ensemble: oct box out of range");
+    }
 
     DevLevel& L = c->ens_level;
     free_level(L);
@@ -198,194 +206,200 @@ int ens_download_state(rh_ctx* c, int what) {
 extern "C" {
 
 int rh_bind_ensemble(rh_ctx* c, int nbox, const rh_mesh* meshes, int mode) {
-    if (!c) return RH_ERR_ARG;
-    if (nbox < 1 || !meshes) return fail(c, RH_ERR_ARG, "an ensemble needs at least one mesh");
-    if (mode != RH_HOST_COHERENT && mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "unknown mode");
-    long long nc = 0, ng = 0;
-    int nlevelmax = 1;
-    for (int b = 0; b < nbox; ++b) {
-        int st = check_box(c, meshes[b], b);
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        if (nbox < 1 || !meshes) return fail(c, RH_ERR_ARG, "an ensemble needs at least one mesh");
+        if (mode != RH_HOST_COHERENT && mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "unknown mode");
+        long long nc = 0, ng = 0;
+        int nlevelmax = 1;
+        for (int b = 0; b < nbox; ++b) {
+            int st = check_box(c, meshes[b], b);
+            if (st) return st;
+            nc += meshes[b].ncoarse;
+            ng += meshes[b].ngridmax;
+            if (meshes[b].nlevelmax > nlevelmax) nlevelmax = meshes[b].nlevelmax;
+        }
+        const long long ncell = nc + 8 * ng;
+        if (ncell > INT_MAX) return fail(c, RH_ERR_ARG, "ensemble has more than 2^31-1 cells");
+        RH_HIP(c, hipSetDevice(c->p.device));
+        unbind(c);
+        c->boxes.resize(nbox);
+        long long coff = 0, goff = 0;
+        for (int b = 0; b < nbox; ++b) {
+            c->boxes[b].mesh = meshes[b];
+            c->boxes[b].coff = coff;
+            c->boxes[b].goff = goff;
+            coff += meshes[b].ncoarse;
+            goff += meshes[b].ngridmax;
+        }
+        rh_mesh& T = c->mesh;
+        T = rh_mesh{};
+        T.ncoarse = (int)nc;
+        T.ngridmax = (int)ng;
+        T.ncell = (int)ncell;
+        T.nvar = 5;
+        T.nlevelmax = nlevelmax;
+        const size_t n = (size_t)5 * ncell;
+        RH_HIP(c, hipMalloc(&c->d_uold, n * sizeof(double)));
+        RH_HIP(c, hipMalloc(&c->d_unew, n * sizeof(double)));
+        c->state_cap = n;
+        RH_HIP(c, hipMalloc(&c->d_boxlen, (size_t)2 * nbox * sizeof(double)));
+        c->ens = true;
+        c->bound = true;
+        c->mode = mode;
+        int st = ens_upload_state(c, RH_UOLD | RH_UNEW);
         if (st) return st;
-        nc += meshes[b].ncoarse;
-        ng += meshes[b].ngridmax;
-        if (meshes[b].nlevelmax > nlevelmax) nlevelmax = meshes[b].nlevelmax;
-    }
-    const long long ncell = nc + 8 * ng;
-    if (ncell > INT_MAX) return fail(c, RH_ERR_ARG, "ensemble has more than 2^31-1 cells");
-    RH_HIP(c, hipSetDevice(c->p.device));
-    unbind(c);
-    c->boxes.resize(nbox);
-    long long coff = 0, goff = 0;
-    for (int b = 0; b < nbox; ++b) {
-        c->boxes[b].mesh = meshes[b];
-        c->boxes[b].coff = coff;
-        c->boxes[b].goff = goff;
-        coff += meshes[b].ncoarse;
-        goff += meshes[b].ngridmax;
-    }
-    rh_mesh& T = c->mesh;
-    T = rh_mesh{};
-    T.ncoarse = (int)nc;
-    T.ngridmax = (int)ng;
-    T.ncell = (int)ncell;
-    T.nvar = 5;
-    T.nlevelmax = nlevelmax;
-    const size_t n = (size_t)5 * ncell;
-    RH_HIP(c, hipMalloc(&c->d_uold, n * sizeof(double)));
-    RH_HIP(c, hipMalloc(&c->d_unew, n * sizeof(double)));
-    c->state_cap = n;
-    RH_HIP(c, hipMalloc(&c->d_boxlen, (size_t)2 * nbox * sizeof(double)));
-    c->ens = true;
-    c->bound = true;
-    c->mode = mode;
-    int st = ens_upload_state(c, RH_UOLD | RH_UNEW);
-    if (st) return st;
-    RH_HIP(c, hipStreamSynchronize(c->stream));
-    return RH_OK;
+        RH_HIP(c, hipStreamSynchronize(c->stream));
+        return RH_OK;
+    });
 }
 
 int rh_run_ensemble(rh_ctx* c, const int* levels, const double* dx, double courant_factor, int nsteps,
                     double* dt_out) {
-    if (!c) return RH_ERR_ARG;
-    if (!c->bound || !c->ens) return fail(c, RH_ERR_ARG, "no ensemble bound (call rh_bind_ensemble first)");
-    if (!levels || !dx) return fail(c, RH_ERR_ARG, "null levels or dx");
-    if (nsteps < 0) return fail(c, RH_ERR_ARG, "nsteps < 0");
-    const int nbox = (int)c->boxes.size();
-    if ((long long)nbox * (nsteps > 0 ? nsteps : 1) > INT_MAX / 2) return fail(c, RH_ERR_ARG, "too many steps");
-    std::vector<double> len(2 * (size_t)nbox);
-    bool all_exact = true;
-    for (int b = 0; b < nbox; ++b) {
-        if (levels[b] < 1 || levels[b] > c->boxes[b].mesh.nlevelmax)
-            return fail(c, RH_ERR_ARG, "box " + std::to_string(b) + ": level out of range");
-        if (!(dx[b] > 0.0)) return fail(c, RH_ERR_ARG, "box " + std::to_string(b) + ": dx must be positive");
-        len[b] = dx[b];
-        len[nbox + b] = exact_inverse(dx[b]);
-        all_exact = all_exact && len[nbox + b] != 0.0;
-    }
-    RH_HIP(c, hipSetDevice(c->p.device));
-    int st;
-    if (c->mode == RH_HOST_COHERENT && (st = ens_upload_state(c, RH_UOLD | RH_UNEW))) return st;
-    RH_HIP(c, hipMemcpyAsync(c->d_boxlen, len.data(), len.size() * sizeof(double), hipMemcpyHostToDevice,
-                             c->stream));
-    if (c->ens_levels.size() != (size_t)nbox || !std::equal(c->ens_levels.begin(), c->ens_levels.end(), levels))
-        if ((st = build_tables(c, levels))) return st;
-    const rh::LevelDev& lv = c->ens_level.dev;
-    const int nk = nbox * (nsteps > 0 ? nsteps : 1);
-    if ((st = ensure_dt(c, nk))) return st;
-    if (nk > c->nkeys) {
-        if (c->d_keys) (void)hipFree(c->d_keys);
-        c->d_keys = nullptr;
-        c->nkeys = 0;
-        RH_HIP(c, hipMalloc(&c->d_keys, (size_t)nk * sizeof(unsigned long long)));
-        c->nkeys = nk;
-    }
-    const rh::MeshDev m = mesh_dev(c);
-    const rh::GasParams gp = gas(c, c->p.gamma);
-    const double dt_init = c->p.boxlen / 1e-10;
-    RH_HIP(c, rh::launch_fill_keys(c->d_keys, nk, dt_init, c->stream));
-    // As in rh_run_uniform: boxes whose levels hold only leaf cells with
-    // same-level neighbours step with uold and unew exchanged, each Godunov
-    // launch taking the next step's per-box Courant keys from the values it
-    // writes; a final copy leaves the last state in both buffers.
-    const bool exchanged = c->variant != 0 && lv.ntgt == 0 && lv.nrec == 0 && lv.nleaf == 8LL * lv.noct &&
-                           lv.noct > 0;
-    if (exchanged) {
-        rh::MeshDev mk = m;
-        for (int k = 0; k < nsteps; ++k) {
-            unsigned long long* keys = c->d_keys + (size_t)k * nbox;
-            double* dts = c->d_dt + (size_t)k * nbox;
-            const bool next = k + 1 < nsteps;
-            if (k == 0) {
-                Timed tm(c, TK_COURANT);
-                RH_HIP(c, rh::launch_courant_boxes(mk, lv, c->p.gamma, courant_factor, keys, c->stream));
-                RH_HIP(c, rh::launch_keys_to_dt(keys, nbox, dts, c->stream));
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        if (!c->bound || !c->ens) return fail(c, RH_ERR_ARG, "no ensemble bound (call rh_bind_ensemble first)");
+        if (!levels || !dx) return fail(c, RH_ERR_ARG, "null levels or dx");
+        if (nsteps < 0) return fail(c, RH_ERR_ARG, "nsteps < 0");
+        const int nbox = (int)c->boxes.size();
+        if ((long long)nbox * (nsteps > 0 ? nsteps : 1) > INT_MAX / 2) return fail(c, RH_ERR_ARG, "too many steps");
+        std::vector<double> len(2 * (size_t)nbox);
+        bool all_exact = true;
+        for (int b = 0; b < nbox; ++b) {
+            if (levels[b] < 1 || levels[b] > c->boxes[b].mesh.nlevelmax)
+                return fail(c, RH_ERR_ARG, "box " + std::to_string(b) + ": level out of range");
+            if (!(dx[b] > 0.0)) return fail(c, RH_ERR_ARG, "box " + std::to_string(b) + ": dx must be positive");
+            len[b] = dx[b];
+            len[nbox + b] = exact_inverse(dx[b]);
+            all_exact = all_exact && len[nbox + b] != 0.0;
+        }
+        RH_HIP(c, hipSetDevice(c->p.device));
+        int st;
+        if (c->mode == RH_HOST_COHERENT && (st = ens_upload_state(c, RH_UOLD | RH_UNEW))) return st;
+        RH_HIP(c, hipMemcpyAsync(c->d_boxlen, len.data(), len.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        if (c->ens_levels.size() != (size_t)nbox || !std::equal(c->ens_levels.begin(), c->ens_levels.end(), levels))
+            if ((st = build_tables(c, levels))) return st;
+        const rh::LevelDev& lv = c->ens_level.dev;
+        const int nk = nbox * (nsteps > 0 ? nsteps : 1);
+        if ((st = ensure_dt(c, nk))) return st;
+        if (nk > c->nkeys) {
+            if (c->d_keys) (void)hipFree(c->d_keys);
+            c->d_keys = nullptr;
+            c->nkeys = 0;
+            RH_HIP(c, hipMalloc(&c->d_keys, (size_t)nk * sizeof(unsigned long long)));
+            c->nkeys = nk;
+        }
+        const rh::MeshDev m = mesh_dev(c);
+        const rh::GasParams gp = gas(c, c->p.gamma);
+        const double dt_init = c->p.boxlen / 1e-10;
+        RH_HIP(c, rh::launch_fill_keys(c->d_keys, nk, dt_init, c->stream));
+        // As in rh_run_uniform: boxes whose levels hold only leaf cells with
+        // same-level neighbours step with uold and unew exchanged, each Godunov
+        // launch taking the next step's per-box Courant keys from the values it
+        // writes; a final copy leaves the last state in both buffers.
+        const bool exchanged = c->variant != 0 && lv.ntgt == 0 && lv.nrec == 0 && lv.nleaf == 8LL * lv.noct &&
+                               lv.noct > 0;
+        if (exchanged) {
+            rh::MeshDev mk = m;
+            for (int k = 0; k < nsteps; ++k) {
+                unsigned long long* keys = c->d_keys + (size_t)k * nbox;
+                double* dts = c->d_dt + (size_t)k * nbox;
+                const bool next = k + 1 < nsteps;
+                if (k == 0) {
+                    Timed tm(c, TK_COURANT);
+                    RH_HIP(c, rh::launch_courant_boxes(mk, lv, c->p.gamma, courant_factor, keys, c->stream));
+                    RH_HIP(c, rh::launch_keys_to_dt(keys, nbox, dts, c->stream));
+                    tm.done();
+                }
+                rh::FusedCourant fc;
+                fc.pre_from_uold = true;
+                fc.keys = next ? keys + nbox : nullptr;
+                fc.courant_factor = courant_factor;
+                {
+                    Timed tm(c, TK_GODUNOV);
+                    hipError_t e = rh::launch_godunov(mk, lv, gp, c->p.riemann, c->p.precision, c->variant, dts, 0.0,
+                                                      all_exact ? 1.0 : 0.0, c->stream, fc);
+                    if (e == hipErrorNotSupported)
+                        return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
+                    if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
+                    tm.done();
+                }
+                if (next) {
+                    Timed tm(c, TK_COURANT);
+                    RH_HIP(c, rh::launch_keys_to_dt(keys + nbox, nbox, dts + nbox, c->stream));
+                    tm.done();
+                }
+                std::swap(mk.uold, mk.unew);
+            }
+            if (nsteps > 0) {
+                Timed tm(c, TK_SET_UOLD);
+                RH_HIP(c, rh::launch_set_unew(mk, lv, c->stream));
                 tm.done();
             }
-            rh::FusedCourant fc;
-            fc.pre_from_uold = true;
-            fc.keys = next ? keys + nbox : nullptr;
-            fc.courant_factor = courant_factor;
+        }
+        // Otherwise set_uold of one step and the Courant step and set_unew of the
+        // next run as one fused pass.
+        for (int k = 0; k < (exchanged ? 0 : nsteps); ++k) {
+            unsigned long long* keys = c->d_keys + (size_t)k * nbox;
+            double* dts = c->d_dt + (size_t)k * nbox;
+            if (k == 0) {
+                {
+                    Timed tm(c, TK_COURANT);
+                    RH_HIP(c, rh::launch_courant_boxes(m, lv, c->p.gamma, courant_factor, keys, c->stream));
+                    RH_HIP(c, rh::launch_keys_to_dt(keys, nbox, dts, c->stream));
+                    tm.done();
+                }
+                Timed tm(c, TK_SET_UNEW);
+                RH_HIP(c, rh::launch_set_unew(m, lv, c->stream));
+                tm.done();
+            }
             {
                 Timed tm(c, TK_GODUNOV);
-                hipError_t e = rh::launch_godunov(mk, lv, gp, c->p.riemann, c->p.precision, c->variant, dts, 0.0,
-                                                  all_exact ? 1.0 : 0.0, c->stream, fc);
+                hipError_t e = rh::launch_godunov(m, lv, gp, c->p.riemann, c->p.precision, c->variant, dts, 0.0,
+                                                  all_exact ? 1.0 : 0.0, c->stream);
                 if (e == hipErrorNotSupported)
                     return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
                 if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
                 tm.done();
             }
-            if (next) {
-                Timed tm(c, TK_COURANT);
-                RH_HIP(c, rh::launch_keys_to_dt(keys + nbox, nbox, dts + nbox, c->stream));
+            if (k + 1 < nsteps) {
+                unsigned long long* next_keys = keys + nbox;
+                Timed tm(c, TK_STEP_JOIN);
+                RH_HIP(c, rh::launch_courant_boxes(m, lv, c->p.gamma, courant_factor, next_keys, c->stream, true));
+                RH_HIP(c, rh::launch_keys_to_dt(next_keys, nbox, dts + nbox, c->stream));
+                tm.done();
+            } else {
+                Timed tm(c, TK_SET_UOLD);
+                RH_HIP(c, rh::launch_set_uold(m, lv, c->stream));
                 tm.done();
             }
-            std::swap(mk.uold, mk.unew);
         }
-        if (nsteps > 0) {
-            Timed tm(c, TK_SET_UOLD);
-            RH_HIP(c, rh::launch_set_unew(mk, lv, c->stream));
-            tm.done();
-        }
-    }
-    // Otherwise set_uold of one step and the Courant step and set_unew of the
-    // next run as one fused pass.
-    for (int k = 0; k < (exchanged ? 0 : nsteps); ++k) {
-        unsigned long long* keys = c->d_keys + (size_t)k * nbox;
-        double* dts = c->d_dt + (size_t)k * nbox;
-        if (k == 0) {
-            {
-                Timed tm(c, TK_COURANT);
-                RH_HIP(c, rh::launch_courant_boxes(m, lv, c->p.gamma, courant_factor, keys, c->stream));
-                RH_HIP(c, rh::launch_keys_to_dt(keys, nbox, dts, c->stream));
-                tm.done();
-            }
-            Timed tm(c, TK_SET_UNEW);
-            RH_HIP(c, rh::launch_set_unew(m, lv, c->stream));
-            tm.done();
-        }
-        {
-            Timed tm(c, TK_GODUNOV);
-            hipError_t e = rh::launch_godunov(m, lv, gp, c->p.riemann, c->p.precision, c->variant, dts, 0.0,
-                                              all_exact ? 1.0 : 0.0, c->stream);
-            if (e == hipErrorNotSupported)
-                return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
-            if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
-            tm.done();
-        }
-        if (k + 1 < nsteps) {
-            unsigned long long* next_keys = keys + nbox;
-            Timed tm(c, TK_STEP_JOIN);
-            RH_HIP(c, rh::launch_courant_boxes(m, lv, c->p.gamma, courant_factor, next_keys, c->stream, true));
-            RH_HIP(c, rh::launch_keys_to_dt(next_keys, nbox, dts + nbox, c->stream));
-            tm.done();
-        } else {
-            Timed tm(c, TK_SET_UOLD);
-            RH_HIP(c, rh::launch_set_uold(m, lv, c->stream));
-            tm.done();
-        }
-    }
-    if (dt_out && nsteps > 0)
-        RH_HIP(c, hipMemcpyAsync(dt_out, c->d_dt, (size_t)nsteps * nbox * sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
-    RH_HIP(c, hipStreamSynchronize(c->stream));
-    if (c->mode == RH_HOST_COHERENT) return ens_download_state(c, RH_UOLD | RH_UNEW);
-    return RH_OK;
+        if (dt_out && nsteps > 0)
+            RH_HIP(c, hipMemcpyAsync(dt_out, c->d_dt, (size_t)nsteps * nbox * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+        RH_HIP(c, hipStreamSynchronize(c->stream));
+        if (c->mode == RH_HOST_COHERENT) return ens_download_state(c, RH_UOLD | RH_UNEW);
+        return RH_OK;
+    });
 }
 
 int rh_ensemble_leaf_cells(rh_ctx* c, const int* levels, long long* nleaf) {
-    if (!c) return RH_ERR_ARG;
-    if (!c->bound || !c->ens) return fail(c, RH_ERR_ARG, "no ensemble bound (call rh_bind_ensemble first)");
-    if (!levels) return fail(c, RH_ERR_ARG, "null levels");
-    const int nbox = (int)c->boxes.size();
-    for (int b = 0; b < nbox; ++b)
-        if (levels[b] < 1 || levels[b] > c->boxes[b].mesh.nlevelmax)
-            return fail(c, RH_ERR_ARG, "box " + std::to_string(b) + ": level out of range");
-    if (c->ens_levels.size() != (size_t)nbox || !std::equal(c->ens_levels.begin(), c->ens_levels.end(), levels)) {
-        int st = build_tables(c, levels);
-        if (st) return st;
-    }
-    if (nleaf) *nleaf = c->ens_level.dev.nleaf;
-    return RH_OK;
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        if (!c->bound || !c->ens) return fail(c, RH_ERR_ARG, "no ensemble bound (call rh_bind_ensemble first)");
+        if (!levels) return fail(c, RH_ERR_ARG, "null levels");
+        const int nbox = (int)c->boxes.size();
+        for (int b = 0; b < nbox; ++b)
+            if (levels[b] < 1 || levels[b] > c->boxes[b].mesh.nlevelmax)
+                return fail(c, RH_ERR_ARG, "box " + std::to_string(b) + ": level out of range");
+        if (c->ens_levels.size() != (size_t)nbox || !std::equal(c->ens_levels.begin(), c->ens_levels.end(), levels)) {
+            int st = build_tables(c, levels);
+            if (st) return st;
+        }
+        if (nleaf) *nleaf = c->ens_level.dev.nleaf;
+        return RH_OK;
+    });
 }
 
 }  // extern "C"

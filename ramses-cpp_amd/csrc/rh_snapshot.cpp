@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "rh_context.h"
+#include "rh_guard.h"
 #include "rh_kernels.h"
 #include "rh_snapshot.h"
 #include "rhydro_snapshot.h"
@@ -409,18 +410,24 @@ int dump_snapshot(rh_ctx* c, const rh_snapshot_tree* t, const rh_snapshot_info* 
 extern "C" {
 
 int rh_dump_snapshot(rh_ctx* ctx, const rh_snapshot_tree* tree, const rh_snapshot_info* info, const char* root) {
-    if (!ctx) return RH_ERR_ARG;
-    return rhi::dump_snapshot(ctx, tree, info, root);
+    return rhi::guard(ctx ? &ctx->err : &rhi::g_create_error, nullptr, [&]() -> int {
+        if (!ctx) return RH_ERR_ARG;
+        return rhi::dump_snapshot(ctx, tree, info, root);
+    });
 }
 
 int rh_write_hydro(rh_ctx* ctx, const rh_snapshot_tree* tree, const rh_snapshot_info* info, const char* path) {
-    if (!ctx || !path) return RH_ERR_ARG;
-    return rhi::write_hydro_file(ctx, tree, info, path);
+    return rhi::guard(ctx ? &ctx->err : &rhi::g_create_error, nullptr, [&]() -> int {
+        if (!ctx || !path) return RH_ERR_ARG;
+        return rhi::write_hydro_file(ctx, tree, info, path);
+    });
 }
 
 int rh_write_amr(const rh_snapshot_tree* tree, const rh_snapshot_info* info, const char* path) {
-    if (!info || !path) return rhi::fail(nullptr, RH_ERR_ARG, "snapshot: info and path must not be NULL");
-    return rhi::write_amr_file(nullptr, tree, info, path);
+    return rhi::guard(&rhi::g_create_error, nullptr, [&]() -> int {
+        if (!info || !path) return rhi::fail(nullptr, RH_ERR_ARG, "snapshot: info and path must not be NULL");
+        return rhi::write_amr_file(nullptr, tree, info, path);
+    });
 }
 
 }  // extern "C"

@@ -288,4 +288,52 @@ int build_level(const rh_mesh& m, const TopoIndex& ti, int level, LevelHost& out
     return RH_OK;
 }
 
+int validate_level(const LevelHost& h, int ngridmax, long long ncell, std::string& err) {
+    auto bad = [&](const std::string& what, long long i, long long v) {
+        err = "level " + std::to_string(h.level) + " table check: " + what + "[" + std::to_string(i) +
+              "] = " + std::to_string(v) + " is out of range";
+        return RH_ERR_TOPOLOGY;
+    };
+    const long long noct = h.noct;
+    const long long nrec = (long long)h.rec_n.size();
+    if ((long long)h.grid.size() != noct || (long long)h.leaf.size() != noct || (long long)h.nbo.size() != 6 * noct)
+        return bad("table size", 0, (long long)h.grid.size());
+    if ((long long)h.rec_nl.size() != nrec || (long long)h.rec_nr.size() != nrec ||
+        (long long)h.rec_flags.size() != nrec || (long long)h.rec_slot.size() != nrec)
+        return bad("record size", 0, nrec);
+    for (long long p = 0; p < noct; ++p)
+        if (h.grid[p] < 1 || h.grid[p] > ngridmax) return bad("grid", p, h.grid[p]);
+    for (long long i = 0; i < 6 * noct; ++i) {
+        const long long q = h.nbo[i];
+        if (q >= noct || (q < 0 && -1 - q >= nrec)) return bad("nbo", i, q);
+    }
+    long long nleaf = 0;
+    for (long long p = 0; p < noct; ++p) nleaf += __builtin_popcount(h.leaf[p]);
+    if (nleaf != h.nleaf) return bad("nleaf", 0, h.nleaf);
+    for (long long r = 0; r < nrec; ++r) {
+        const int fl = h.rec_flags[r];
+        if (fl != REC_REFINED && fl != REC_BOUNDARY && fl != REC_REFLUX) return bad("rec_flags", r, fl);
+        // A boundary record's cell is the boundary code itself (<= 0); the
+        // kernels read the others' cells, and a reflux record's nl/nr, from
+        // the state (an nl/nr <= 0 falls back to the neighbour's own state).
+        if (fl == REC_BOUNDARY ? h.rec_n[r] > 0 : (h.rec_n[r] < 1 || h.rec_n[r] > ncell))
+            return bad("rec_n", r, h.rec_n[r]);
+        if (h.rec_nl[r] > ncell) return bad("rec_nl", r, h.rec_nl[r]);
+        if (h.rec_nr[r] > ncell) return bad("rec_nr", r, h.rec_nr[r]);
+        const int s = h.rec_slot[r];
+        if (fl == REC_REFLUX ? (s < 0 || (long long)s + 4 > h.nslots) : s != -1) return bad("rec_slot", r, s);
+    }
+    const long long ntgt = (long long)h.tgt_cell.size();
+    if ((long long)h.tgt_off.size() != ntgt + 1 || h.tgt_off[0] != 0 ||
+        h.tgt_off[ntgt] != (long long)h.tgt_slot.size())
+        return bad("tgt_off", ntgt, h.tgt_off.empty() ? -1 : h.tgt_off.back());
+    for (long long t = 0; t < ntgt; ++t) {
+        if (h.tgt_cell[t] < 1 || h.tgt_cell[t] > ncell) return bad("tgt_cell", t, h.tgt_cell[t]);
+        if (h.tgt_off[t + 1] < h.tgt_off[t]) return bad("tgt_off", t + 1, h.tgt_off[t + 1]);
+    }
+    for (size_t i = 0; i < h.tgt_slot.size(); ++i)
+        if (h.tgt_slot[i] < 0 || h.tgt_slot[i] >= h.nslots) return bad("tgt_slot", (long long)i, h.tgt_slot[i]);
+    return RH_OK;
+}
+
 }  // namespace rh

@@ -59,6 +59,14 @@ int build_topo_index(const rh_mesh& m, TopoIndex& ti, std::string& err);
 // branch) with a message in err.
 int build_level(const rh_mesh& m, const TopoIndex& ti, int level, LevelHost& out, std::string& err);
 
+// Checks every index the kernels follow from a level's tables before they
+// are uploaded: grid ids within [1, ngridmax]; each nbo entry an oct position
+// of the level or a record code; record cells at most ncell (a boundary code
+// <= 0 only where the kernels read it as one); reflux slots inside nslots;
+// target lists well formed.  A table that fails never reaches a launch:
+// RH_ERR_TOPOLOGY with the first offending entry in err.
+int validate_level(const LevelHost& h, int ngridmax, long long ncell, std::string& err);
+
 // Neighbour cells of a cell (1-based), as AmrGrid::get_nbor_cells /
 // get_nbor_cells_coarse return them.
 void nbor_cells_of_cell(const rh_mesh& m, int icell, int icn[6]);

@@ -15,7 +15,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librhydro.so")
 
-RH_OK, RH_ERR_ARG, RH_ERR_HIP, RH_ERR_UNSUPPORTED, RH_ERR_TOPOLOGY, RH_ERR_NO_DEVICE = range(6)
+RH_OK, RH_ERR_ARG, RH_ERR_HIP, RH_ERR_UNSUPPORTED, RH_ERR_TOPOLOGY, RH_ERR_NO_DEVICE, RH_ERR_HOST = range(7)
 RIEMANN = {"llf": 0, "hll": 1, "hllc": 2, "exact": 3, "acoustic": 4}
 RH_PREC_FP64, RH_PREC_MIXED = 0, 1
 RH_HOST_COHERENT, RH_DEVICE_RESIDENT = 0, 1
