@@ -119,6 +119,17 @@ int rh_set_uold(rh_ctx* ctx, int ilevel);
  * courant_factor) (HydroSolver.cpp:878-977); the result goes to *dt_out. */
 int rh_courant(rh_ctx* ctx, int ilevel, double dx, double gamma, double courant_factor, double* dt_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid it): the Courant step of
+ * rh_courant, with its block minima and result in a static device array of
+ * the library instead of buffers the context allocates.  It calls no device
+ * allocation or free.  Limits: the level may hold at most 8,388,608 octs
+ * (RH_ERR_ARG beyond); its tables must already be built for the bound mesh
+ * by an earlier call on that level (rh_courant, rh_set_unew,
+ * rh_godunov_fine), else RH_ERR_ARG.  Every context on one GPU shares the
+ * array: calls that overlap on one GPU race. */
+int rh_courant_noalloc(rh_ctx* ctx, int ilevel, double dx, double gamma, double courant_factor, double* dt_out);
+
 /* Replaces HydroSolver::interpol_hydro(u1[7][64], u2[8][64])
  * (HydroSolver.cpp:734-876), the prolongation used when refining.  u1 rows
  * are the coarse cell and its 6 neighbours; u2 rows the 8 children.  Runs

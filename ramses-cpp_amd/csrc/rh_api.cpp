@@ -340,6 +340,10 @@ int rh_create(const rh_params* params, rh_ctx** out) {
         if (e != hipSuccess) return fail(nullptr, RH_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
         e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) return fail(nullptr, RH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        // Once per device (cached after the first context): the address of
+        // the allocation-free Courant step's scratch array on this GPU.
+        c->scratch_err = rh::noalloc_scratch(params->device, &c->scratch);
+        if (c->scratch_err != hipSuccess) c->scratch = nullptr;
         *out = c.release();
         return RH_OK;
     });
@@ -499,6 +503,36 @@ int rh_courant(rh_ctx* c, int ilevel, double dx, double gamma, double courant_fa
         if ((st = ensure_dt(c, 1))) return st;
         if ((st = do_courant(c, lv, dx, gamma, courant_factor, c->d_dt))) return st;
         RH_HIP(c, hipMemcpyAsync(dt_out, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        RH_HIP(c, hipStreamSynchronize(c->stream));
+        return RH_OK;
+    });
+}
+
+int rh_courant_noalloc(rh_ctx* c, int ilevel, double dx, double gamma, double courant_factor, double* dt_out) {
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (!dt_out) return fail(c, RH_ERR_ARG, "null dt_out");
+        if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+        if (!c->scratch) return hip_fail(c, c->scratch_err, "scratch address of the allocation-free Courant step");
+        // Building a level's tables allocates, so they must already be current
+        // for the bound mesh (built by an earlier call on this level).
+        if (ilevel >= (int)c->levels.size() || c->levels[ilevel].epoch != c->mesh.mesh_epoch ||
+            c->levels[ilevel].ngridmax != c->mesh.ngridmax || c->levels[ilevel].ncell != c->mesh.ncell)
+            return fail(c, RH_ERR_ARG,
+                        "level tables not built for the bound mesh: call rh_courant, rh_set_unew or rh_godunov_fine "
+                        "on this level first");
+        const rh::LevelDev* lv = &c->levels[ilevel].dev;
+        if (lv->noct > rh::NOALLOC_COURANT_MAX_OCTS)
+            return fail(c, RH_ERR_ARG,
+                        "level has " + std::to_string(lv->noct) + " octs; the allocation-free Courant step takes at most " +
+                            std::to_string(rh::NOALLOC_COURANT_MAX_OCTS));
+        if ((st = pre_call(c))) return st;
+        double* part = c->scratch;
+        double* d_dt = c->scratch + rh::NOALLOC_COURANT_PARTS;
+        RH_HIP(c, rh::launch_courant(mesh_dev(c), *lv, dx, gamma, courant_factor, c->p.boxlen / 1e-10, part,
+                                     rh::NOALLOC_COURANT_PARTS, d_dt, c->stream));
+        RH_HIP(c, hipMemcpyAsync(dt_out, d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         RH_HIP(c, hipStreamSynchronize(c->stream));
         return RH_OK;
     });

@@ -60,6 +60,11 @@ struct rh_ctx {
     double* d_u2 = nullptr;
     int interpol_cap = 0;
     int variant = 2;  // Godunov kernel variant (rh_set_kernel_variant)
+    // Device address of the allocation-free Courant step's scratch array on
+    // this context's GPU (looked up by rh_create), or null with the lookup's
+    // error in scratch_err.
+    double* scratch = nullptr;
+    hipError_t scratch_err = hipSuccess;
     bool timers = false;
     double t_ms[rhi::TK_N] = {0, 0, 0, 0, 0};
     double table_ms = 0.0;  // host time spent rebuilding level tables

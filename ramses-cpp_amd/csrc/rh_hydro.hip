@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 #include "rh_kernels.h"
@@ -1518,6 +1519,7 @@ inline int blocks_for(int noct) { return (noct + OCTS_PER_BLOCK - 1) / OCTS_PER_
 }  // namespace
 
 int courant_blocks(const LevelDev& lv) { return stream_blocks(lv.noct); }
+static_assert(NOALLOC_COURANT_MAX_OCTS == (long long)NOALLOC_COURANT_PARTS * SBLOCK, "octs per Courant block");
 
 hipError_t launch_set_unew(const MeshDev& m, const LevelDev& lv, hipStream_t stream) {
     if (lv.noct == 0) return hipSuccess;
@@ -1723,6 +1725,44 @@ hipError_t launch_pack_hydro(const MeshDev& m, const int* d_grids, int n, int il
     hipLaunchKernelGGL(k_pack_hydro, dim3((threads + 255) / 256), dim3(256), 0, stream, m, d_grids, n, il, nbnd,
                        gamma, d_out);
     return hipGetLastError();
+}
+
+}  // namespace rh
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free Courant step (rh_courant_noalloc): block
+// minima, then the time step.  Every call on one GPU shares this array, so
+// calls that overlap on one GPU race.  Nothing in it has to start at zero:
+// k_courant_partial writes every block minimum that k_courant_final reads,
+// and k_courant_final writes the time step.
+static __device__ __align__(256) double scratch[rh::NOALLOC_SCRATCH_DOUBLES];
+
+namespace rh {
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, double** ptr) {
+    if (!ptr || device < 0 || device >= SCRATCH_MAX_DEVICES) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        // The array's name is a host-side stand-in; its address on this
+        // device is what kernels and copies must be given.
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = (double*)p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 }  // namespace rh

@@ -72,6 +72,17 @@ hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, doubl
                           double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream,
                           bool set_unew = false, const AmrTimesArgs& at = AmrTimesArgs{});
 int courant_blocks(const LevelDev& lv);
+// Allocation-free Courant step: its block minima and the resulting time step
+// live in a file-scope device array (one copy per GPU, held until the process
+// ends) of NOALLOC_SCRATCH_DOUBLES doubles -- NOALLOC_COURANT_PARTS block
+// minima, then the time step -- so a level of at most
+// NOALLOC_COURANT_PARTS * 256 octs needs no device allocation.
+constexpr int NOALLOC_COURANT_PARTS = 32768;
+constexpr int NOALLOC_SCRATCH_DOUBLES = NOALLOC_COURANT_PARTS + 32;
+constexpr long long NOALLOC_COURANT_MAX_OCTS = (long long)NOALLOC_COURANT_PARTS * 256;
+// Device address of that array on `device`, looked up once per device (with
+// the device current) and cached; later calls only read the cache.
+hipError_t noalloc_scratch(int device, double** ptr);
 // set_uold of one step, then the Courant minimum (into *d_dt) and set_unew of
 // the next, fused into one pass over the level; same d_part requirement.
 hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,

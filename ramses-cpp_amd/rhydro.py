@@ -126,6 +126,7 @@ def lib() -> C.CDLL:
         L.rh_set_uold.argtypes = [C.c_void_p, C.c_int]
         L.rh_godunov_fine.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.rh_courant.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, _DP]
+        L.rh_courant_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, _DP]
         L.rh_interpol_hydro.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
         L.rh_sync.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.rh_last_error.argtypes = [C.c_void_p]
@@ -170,6 +171,7 @@ def lib() -> C.CDLL:
 
 EXPORTED = [
     "rh_create", "rh_destroy", "rh_bind", "rh_set_unew", "rh_godunov_fine", "rh_set_uold", "rh_courant",
+    "rh_courant_noalloc",
     "rh_interpol_hydro", "rh_sync", "rh_last_error", "rh_run_uniform", "rh_uniform_ngrid", "rh_build_uniform",
     "rh_init_regions", "rh_init_lognormal", "rh_timers_enable", "rh_timers_read", "rh_timers_reset",
     "rh_level_leaf_cells", "rh_set_kernel_variant", "rh_get_kernel_variant", "rh_autotune", "rh_init_shock_tube",
@@ -497,6 +499,11 @@ class Solver:
     def courant(self, ilevel, dx, gamma, courant_factor) -> float:
         out = C.c_double()
         self._chk(lib().rh_courant(self._ctx, ilevel, dx, gamma, courant_factor, C.byref(out)))
+        return out.value
+
+    def courant_noalloc(self, ilevel, dx, gamma, courant_factor) -> float:
+        out = C.c_double()
+        self._chk(lib().rh_courant_noalloc(self._ctx, ilevel, dx, gamma, courant_factor, C.byref(out)))
         return out.value
 
     def run_uniform(self, ilevel, dx, courant_factor, nsteps) -> np.ndarray:
