@@ -37,6 +37,7 @@
 #include "rh_context.h"
 #include "rh_guard.h"
 #include "rh_kernels.h"
+#include "rh_pool.h"
 #include "rh_tables.h"
 #include "rhydro.h"
 #include "rhydro_amr.h"
@@ -46,6 +47,7 @@
 
 struct rh_amr {
     rh_ctx* ctx = nullptr;
+    bool pool_user = false;  // holds a reference to the worker pool
     rh_params hp{};
     rh_amr_params ap{};
     // AmrGrid (AmrGrid.hpp:15-33).
@@ -419,87 +421,6 @@ bool neighbours_refined_by_search(const rh_amr* a, int icell) {
 }
 
 // ---- host parallelism ------------------------------------------------------
-
-// Worker threads kept for the life of the process: a flag pass is a few
-// hundred microseconds of work, comparable to starting and joining threads,
-// so the passes share one pool instead of spawning their own.  run() hands
-// out chunks [k*chunk, (k+1)*chunk) through an atomic counter; the caller
-// works too and returns once every chunk is done.  One pass at a time (the
-// driver is single-threaded).
-class HostPool {
-  public:
-    explicit HostPool(int nworkers) {
-        for (int i = 0; i < nworkers; ++i) std::thread([this] { worker(); }).detach();
-    }
-    void run(int nchunks, const std::function<void(int)>& fn) {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            // A worker still leaving the previous pass may take a chunk of
-            // this one as soon as next_ is reset: everything it reads is set
-            // first (the chunk count never changes).
-            fn_.store(&fn);
-            nchunks_.store(nchunks);
-            left_.store(nchunks);
-            next_.store(0);
-            ++gen_;
-        }
-        cv_.notify_all();
-        work();
-        std::unique_lock<std::mutex> lk(m_);
-        done_.wait(lk, [this] { return left_.load() == 0; });
-        fn_.store(nullptr);
-    }
-
-  private:
-    void work() {
-        for (int k; (k = next_.fetch_add(1)) < nchunks_.load();) {
-            (*fn_.load())(k);
-            if (left_.fetch_sub(1) == 1) {
-                std::lock_guard<std::mutex> lk(m_);
-                done_.notify_all();
-            }
-        }
-    }
-    void worker() {
-        unsigned long long seen = 0;
-        for (;;) {
-            {
-                std::unique_lock<std::mutex> lk(m_);
-                cv_.wait(lk, [&] { return gen_ != seen; });
-                seen = gen_;
-                if (!fn_.load()) continue;
-            }
-            work();
-        }
-    }
-    std::mutex m_;
-    std::condition_variable cv_, done_;
-    std::atomic<const std::function<void(int)>*> fn_{nullptr};
-    std::atomic<int> nchunks_{0}, next_{0}, left_{0};
-    unsigned long long gen_ = 0;
-};
-
-// Runs fn(i) for i in [0, n) on the host's cores (the flag passes touch each
-// oct's own cells only, or defer their writes, so octs are independent).
-template <class Fn> void parallel_for(int n, Fn fn, int min_parallel = 4096) {
-    static const int nthreads = [] {
-        unsigned h = std::thread::hardware_concurrency();
-        return (int)std::max(1u, std::min(h == 0 ? 1u : h, 16u));
-    }();
-    if (n < min_parallel || nthreads == 1) {
-        for (int i = 0; i < n; ++i) fn(i);
-        return;
-    }
-    // Never destroyed: its detached workers wait on it until the process ends.
-    static HostPool* pool = new HostPool(nthreads - 1);
-    const int nchunks = 4 * nthreads;
-    const int chunk = (n + nchunks - 1) / nchunks;
-    const std::function<void(int)> body = [&](int k) {
-        const int lo = k * chunk, hi = std::min(n, lo + chunk);
-        for (int i = lo; i < hi; ++i) fn(i);
-    };
-    pool->run(nchunks, body);
-}
 
 // The grids of a level list, in list order: walked again only after the
 // list was edited (every edit goes through tree_changed).  The reference
@@ -1056,6 +977,7 @@ void destroy(rh_amr* a) {
             if (p) (void)hipFree(p);
         rh_destroy(a->ctx);
     }
+    if (a->pool_user) pool_release();
     delete a;
 }
 
@@ -1073,6 +995,16 @@ int rh_amr_create(const rh_params* hydro, const rh_amr_params* ap, rh_amr** out)
         const long long ncell = (long long)ap->nx * ap->ny * ap->nz + 8LL * ap->ngridmax;
         if (ncell > INT_MAX / 5) return RH_ERR_ARG;
         rh_amr* a = new rh_amr();
+        // Destroys the run on every early return or exception until it is
+        // handed out.
+        struct Owner {
+            rh_amr* a;
+            ~Owner() {
+                if (a) destroy(a);
+            }
+        } own{a};
+        pool_acquire();
+        a->pool_user = true;
         a->hp = *hydro;
         a->ap = *ap;
         a->nx = ap->nx;
@@ -1093,10 +1025,7 @@ int rh_amr_create(const rh_params* hydro, const rh_amr_params* ap, rh_amr** out)
             }
         a->ap.reg_type = nullptr;  // the run keeps its own copy
         int st = rh_create(hydro, &a->ctx);
-        if (st != RH_OK) {
-            delete a;
-            return st;
-        }
+        if (st != RH_OK) return st;
         allocate(a);
         refresh_mesh(a);
         st = rh_bind(a->ctx, &a->mesh, RH_DEVICE_RESIDENT);
@@ -1114,9 +1043,9 @@ int rh_amr_create(const rh_params* hydro, const rh_amr_params* ap, rh_amr** out)
         if (st == RH_OK) st = initialize(a);
         if (st != RH_OK) {
             g_create_error = a->err.empty() ? rh_last_error(a->ctx) : a->err;
-            destroy(a);
             return st;
         }
+        own.a = nullptr;
         *out = a;
         return RH_OK;
     });
