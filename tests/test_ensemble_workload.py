@@ -54,3 +54,37 @@ def test_axis_aligned_tube_splits_at_the_plane():
     rho = m.state("uold")[0, cells - 1]
     np.testing.assert_array_equal(rho, np.where(cen[:, 0] < 0.5, 1.0, 0.125))
     assert (cen[:, 0] < 0.5).sum() == (cen[:, 0] > 0.5).sum()
+
+
+def test_mixed_ensemble_sedov_boxes():
+    """box_ensemble: about half the boxes are Sedov blasts with the energy
+    deposited in the 8 cells around the centre (p = (gamma-1) E / (8 h^3),
+    Initializer.cpp:87-94), the rest rho = 1, p = 1e-5 at rest; the
+    shock-tube boxes are drawn as before and sedov_share = 0 gives exactly
+    shock_tube_ensemble's boxes."""
+    boxes = rhydro.box_ensemble(32, seed=21)
+    kinds = [bx.kind for bx in boxes]
+    assert 8 <= kinds.count("sedov") <= 24 and "sod" in kinds
+    gamma = 1.4
+    for bx in boxes:
+        if bx.kind != "sedov":
+            continue
+        assert 0.1 <= bx.energy <= 1.0
+        m = bx.mesh
+        h = 1.0 / (bx.nx * 2 ** (bx.level - 1))
+        cells, cen = cell_centres(m, bx.level)
+        u = m.state("uold")[:, cells - 1]
+        hot = ((cen - 0.5) ** 2).sum(1) < (1.5 * h) ** 2
+        assert hot.sum() == 8
+        np.testing.assert_array_equal(u[0], 1.0)
+        np.testing.assert_array_equal(u[1:4], 0.0)
+        p_hot = (gamma - 1.0) * bx.energy / (8 * h ** 3)
+        np.testing.assert_allclose(u[4, hot] * (gamma - 1.0), p_hot, rtol=1e-14)
+        np.testing.assert_allclose(u[4, ~hot] * (gamma - 1.0), 1e-5, rtol=1e-14)
+        # the deposited energy is E (up to the background)
+        np.testing.assert_allclose(u[4, hot].sum() * h ** 3, bx.energy, rtol=1e-12)
+    a = rhydro.box_ensemble(12, seed=5, sedov_share=0.0)
+    b = rhydro.shock_tube_ensemble(12, seed=5)
+    for x, y in zip(a, b):
+        assert x.kind == y.kind == "sod"
+        np.testing.assert_array_equal(x.mesh.uold, y.mesh.uold)

@@ -107,3 +107,39 @@ def test_exhausted_grids_are_reported():
     with pytest.raises(rhydro.RhError):
         g.step(4)
     g.close()
+
+
+def test_sedov_amr_run_at_benchmark_levels():
+    """The benchmark's adaptive Sedov run at its own levels (64^3 base at
+    reference level 7, refined to level 10, err_grad 0.1): the initial mesh
+    and the first two coarse steps (the blast reaches level 10 in the first)
+    identical to the oracle's restatement."""
+    kw = dict(levelmin=7, levelmax=10, ngridmax=300000, err_grad_d=0.1, err_grad_p=0.1, riemann="hllc",
+              regions=rhydro.sedov_regions(7))
+    print("case amr run sedov levels 7-10", flush=True)
+    g, o = _pair(**kw)
+    _compare(g.state(), o.state(), "initial mesh")
+    for k in range(2):
+        g.step(1)
+        assert o.step(1) == 0
+        _compare(g.state(), o.state(), f"coarse step {k + 1}")
+    assert g.state()["numbl"][10] > 0
+    g.close()
+    o.close()
+
+
+def test_kelvin_helmholtz_amr_run_scaled():
+    """The adaptive Kelvin-Helmholtz slab of the benchmark two levels
+    coarser (32^3 base at reference level 6, refined to level 9 on density
+    gradients, err_grad_d 0.2): one coarse step (8 fine steps on the finest
+    level) identical to the oracle's restatement, in the FP64 form."""
+    kw = dict(levelmin=6, levelmax=9, ngridmax=250000, err_grad_d=0.2, ic="kh", kh_amp=0.01, riemann="hllc")
+    print("case amr run kh levels 6-9", flush=True)
+    g, o = _pair(**kw)
+    _compare(g.state(), o.state(), "initial mesh")
+    g.step(1)
+    assert o.step(1) == 0
+    _compare(g.state(), o.state(), "coarse step 1")
+    assert g.state()["numbl"][9] > 0
+    g.close()
+    o.close()

@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 import rhydro
-from helpers import assert_same, clone, oracle_params
+from helpers import VARIANTS, assert_same, clone, oracle_params
 
 pytestmark = pytest.mark.gpu
 
@@ -32,7 +32,7 @@ def _boxes(seed, shapes=SHAPES, gamma=1.4):
     return out
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("riemann", ["llf", "hll", "hllc", "acoustic"])
 def test_ensemble_fp64_matches_oracle_per_box(riemann, variant):
     boxes = _boxes(100)
@@ -141,3 +141,26 @@ def test_single_mesh_non_power_of_two_dx():
         s.close()
         assert_same(dtg, dto, "dt")
         assert_same(mg.uold, mo.uold, "uold")
+
+
+def test_benchmark_ensemble_96_boxes_fp64_step_per_box():
+    """The benchmark's batched workload: 96 independent periodic boxes of
+    16^3-64^3 cells, shock tubes and Sedov blasts (seed 20261202), advanced
+    together by one FP64 step, each box with its own time step.  Every box
+    equals the oracle run on it alone, time step included."""
+    boxes = rhydro.box_ensemble(96, seed=20261202)
+    assert {b.kind for b in boxes} == {"sod", "sedov"}
+    meshes = [clone(b.mesh) for b in boxes]
+    print("case ensemble 96 boxes fp64", flush=True)
+    s = rhydro.Solver(gamma=1.4, slope_type=1, riemann="hllc")
+    s.bind_ensemble(meshes)
+    dts = s.run_ensemble([b.level for b in boxes], [b.dx for b in boxes], 0.8, 1)
+    s.sync_to_host()
+    p = oracle_params(s)
+    s.close()
+    for k, b in enumerate(boxes):
+        mo = clone(b.mesh)
+        dto = oracle.OracleMesh(mo).run_uniform(p, b.level, b.dx, 0.8, 1)
+        assert_same(dts[:, k], dto, f"box {k} ({b.kind}) dt")
+        assert_same(meshes[k].uold, mo.uold, f"box {k} ({b.kind}) uold")
+        assert_same(meshes[k].unew, mo.unew, f"box {k} ({b.kind}) unew")

@@ -46,11 +46,14 @@ def _interior_cells(m, nsample):
     return cells.reshape(-1), int(inner.sum())
 
 
-def test_full_level_fp64_step_matches_oracle(level9):
+@pytest.mark.parametrize("riemann,slope", [("hllc", 1), ("hll", 1), ("hllc", 2), ("hll", 2)])
+def test_full_level_fp64_step_matches_oracle(level9, riemann, slope):
+    """The benchmark workload's four variants (HLL and HLLC, slope_type 1 and
+    2) at its full size."""
     mg, mo = clone(level9), clone(level9)
-    s = rhydro.Solver(gamma=GAMMA, slope_type=1, riemann="hllc")
+    s = rhydro.Solver(gamma=GAMMA, slope_type=slope, riemann=riemann)
     s.bind(mg)
-    print(f"case full-size fp64: {2**21} octs at level {LEVEL}", flush=True)
+    print(f"case full-size fp64 {riemann} slope {slope}: {2**21} octs at level {LEVEL}", flush=True)
     dt = s.courant(LEVEL, DX, GAMMA, 0.8)
     s.set_unew(LEVEL)
     s.godunov_fine(LEVEL, dt, DX)
@@ -60,7 +63,8 @@ def test_full_level_fp64_step_matches_oracle(level9):
     om = oracle.OracleMesh(mo)
     assert om.courant(LEVEL, DX, GAMMA, 0.8) == dt
     om.set_unew(LEVEL)
-    om.godunov_fine_sample(oracle.params(GAMMA, 1, 2, 1, 0, 1.0), LEVEL, dt, DX, SAMPLE_OCTS)
+    om.godunov_fine_sample(oracle.params(GAMMA, slope, rhydro.resolve_riemann(riemann), 1, 0, 1.0), LEVEL, dt, DX,
+                           SAMPLE_OCTS)
 
     cells, ninner = _interior_cells(level9, SAMPLE_OCTS)
     assert ninner == 12600
