@@ -28,29 +28,37 @@ METRIC = "hydro cell-updates/s on 1 MI355X, FP64 vs mixed+batched; % of HBM roof
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 80  # read uold (5 x FP64) + write unew (5 x FP64)
 
-# HBM bytes per launch of the Godunov kernel, measured with rocprofv3 --pmc
-# FETCH_SIZE and --pmc WRITE_SIZE runs of this script (each counter scaled by
-# what it reads for k_set_unew, a copy of known size, in the same run); keyed
-# by (level, riemann, slope_type, kernel, form), kernel "oct" for the
-# oct-in-lanes variants (profiled at 8 octs per workgroup, the variant the
-# autotuner keeps here, on the launches of the timed steps, which read the
-# pre-update state from uold and end with the next step's Courant
-# candidates; the 16-oct variant reads the same counts) and "cell" for the
-# per-cell one.  Counters cannot
-# be read while timing, so only profiled workloads report a figure.
-PROFILED_TRAFFIC = {
-    (9, "hllc", 1, "oct", "fp64"): 1.665e9,
-    (9, "hllc", 1, "oct", "mixed"): 1.668e9,
-}
+# Counter figures of the Godunov kernel from rocprofv3 runs of this script
+# (profiles/), keyed by (level, riemann, slope_type, kernel variant, form):
+#   traffic  HBM bytes per launch, from --pmc FETCH_SIZE and --pmc WRITE_SIZE
+#            runs, each counter scaled by what it reads for k_set_unew, a copy
+#            of known size, in the same run;
+#   busy     VALU utilisation (SQ_ACTIVE_INST_VALU x 4 / SIMDs / busy cycles);
+#   insts_per_cell  VALU instructions per wave = per cell (one cell per lane),
+#            SQ_INSTS_VALU / SQ_WAVES.
+# Counters cannot be read while timing.  Each entry records the hash of the
+# kernel sources it was measured on (KERNEL_SOURCES); a build whose sources
+# hash differently reports null instead of a stale figure.
+KERNEL_SOURCES = ["ramses-cpp_amd/csrc/rh_hydro.hip", "ramses-cpp_amd/csrc/rh_physics.h",
+                  "ramses-cpp_amd/csrc/rh_kernels.h", "ramses-cpp_amd/Makefile"]
+PROFILED = {}
 
-# VALU utilisation of the same launches from an SQ/GRBM counter run
-# (SQ_ACTIVE_INST_VALU x 4 / SIMDs / active cycles per XCD) and VALU
-# instructions per wave (= per cell: 64 cells per wave) from SQ_INSTS_VALU /
-# SQ_WAVES, same key.
-PROFILED_VALU = {
-    (9, "hllc", 1, "oct", "fp64"): {"busy": 0.77, "insts_per_cell": 5985},
-    (9, "hllc", 1, "oct", "mixed"): {"busy": 0.72, "insts_per_cell": 4229},
-}
+
+def kernel_source_hash() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def profiled(level, riemann, slope, variant, form, what):
+    """A counter figure of this workload and kernel build, or None."""
+    e = PROFILED.get((level, riemann, slope, variant, form))
+    if e is None or e["sources"] != kernel_source_hash():
+        return None
+    return e[what]
 
 
 def reference_arithmetic(rate: float, riemann: str, slope: int):
@@ -66,8 +74,6 @@ def reference_arithmetic(rate: float, riemann: str, slope: int):
             "frac": tflops / FP64_PEAK_TFLOPS, "counted_by": "oracle/flopcount.cpp"}
 
 
-def kernel_family(variant: int) -> str:
-    return "cell" if variant == 0 else "oct"
 PROFILE_NOTE = "profiles/godunov_level9_256cubed_hllc_minmod.md (rocprofv3 counter runs of this script)"
 
 # FP64 operations (add/sub, mul, div, sqrt) of the reference's godunov_fine per
@@ -106,13 +112,15 @@ def state_errors(A, B, gamma, np):
 
 
 def measure_ensemble(args, rhydro, np):
-    """The batched form on the shock-tube ensemble: `boxes` independent
-    periodic boxes of 16-64 cells a side, each with its own level, cell width
-    and time step, advanced together (rh_run_ensemble), in the mixed and FP64
-    forms; then the same boxes one after another through rh_run_uniform (the
-    unbatched alternative), whose final states must equal the batched ones."""
+    """The batched form on the box ensemble: `boxes` independent periodic
+    boxes of 16-64 cells a side, shock tubes and Sedov blasts, each with its
+    own level, cell width and time step, advanced together (rh_run_ensemble),
+    in the mixed and FP64 forms; then the same boxes one after another through
+    rh_run_uniform (the unbatched alternative), whose final states must equal
+    the batched ones."""
     gamma, cf = 1.4, 0.8
-    boxes = rhydro.shock_tube_ensemble(args.boxes, seed=args.seed, gamma=gamma)
+    boxes = rhydro.box_ensemble(args.boxes, seed=args.seed, gamma=gamma)
+    nsedov = sum(b.kind == "sedov" for b in boxes)
     meshes = [b.mesh for b in boxes]
     levels = [b.level for b in boxes]
     dxs = [b.dx for b in boxes]
@@ -182,10 +190,13 @@ def measure_ensemble(args, rhydro, np):
             err_summary[f"{name}_{norm}"] = {"max": float(vals.max()), "median": float(np.median(vals)),
                                              "worst_box": int(vals.argmax())}
     return {
-        "workload": f"shock-tube ensemble: {args.boxes} periodic cubic boxes, sides {sides[0]}-{sides[-1]} cells "
-                    f"(log-uniform, nx*2^(L-1) with nx in 1,3,5,7; seed {args.seed}), {nleaf} leaf cells, "
-                    "random plane normal, rho and p in [0.1, 1] per side, gamma=1.4, HLLC, slope_type 1, "
-                    "own Courant time step per box",
+        "workload": f"box ensemble: {args.boxes} periodic cubic boxes, sides {sides[0]}-{sides[-1]} cells "
+                    f"(log-uniform, nx*2^(L-1) with nx in 1,3,5,7; seed {args.seed}), {nleaf} leaf cells; "
+                    f"{args.boxes - nsedov} shock tubes (random plane normal, rho and p in [0.1, 1] per side) and "
+                    f"{nsedov} Sedov blasts (energy in [0.1, 1] in the 8 central cells, rho=1, p=1e-5); gamma=1.4, "
+                    "HLLC, slope_type 1, own Courant time step per box",
+        "boxes_sod": args.boxes - nsedov,
+        "boxes_sedov": nsedov,
         "cells": int(nleaf),
         "value_mixed": out["mixed"]["value"],
         "value_fp64": out["fp64"]["value"],
@@ -367,7 +378,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="skip the mixed-precision form")
-    ap.add_argument("--boxes", type=int, default=96, help="boxes of the shock-tube ensemble (0: skip it)")
+    ap.add_argument("--boxes", type=int, default=96, help="boxes of the shock-tube and Sedov ensemble (0: skip it)")
     ap.add_argument("--no-variants", action="store_true", help="skip the HLL / slope-2 variants of the workload")
     ap.add_argument("--no-sedov", action="store_true", help="skip the 64^3 Sedov run")
     ap.add_argument("--only", choices=["ensemble", "amr_sedov", "amr_kh", "sedov64"],
@@ -468,10 +479,9 @@ def main() -> int:
             "kernel_ms": mixed["kernel_ms"],
             "autotune": mixed["autotune"],
             "roofline_frac": mixed["achieved"] / HBM_PEAK_GBS,
-            "traffic": PROFILED_TRAFFIC.get((level, args.riemann, args.slope, kernel_family(mixed["autotune"]["chosen_variant"]),
-                                             "mixed")),
-            "valu": PROFILED_VALU.get((level, args.riemann, args.slope, kernel_family(mixed["autotune"]["chosen_variant"]),
-                                       "mixed")),
+            "traffic": profiled(level, args.riemann, args.slope, mixed["autotune"]["chosen_variant"], "mixed",
+                                "traffic"),
+            "valu": profiled(level, args.riemann, args.slope, mixed["autotune"]["chosen_variant"], "mixed", "valu"),
             "speedup_vs_fp64": fp64["wall"] / mixed["wall"],
             "error_vs_fp64_one_step": err1,
             "error_vs_fp64_after_steps": errn,
@@ -568,8 +578,9 @@ def main() -> int:
             "slope_type": args.slope,
             "kernel_ms": fp64["kernel_ms"],
             "autotune": fp64["autotune"],
-            "valu": PROFILED_VALU.get((level, args.riemann, args.slope, kernel_family(fp64["autotune"]["chosen_variant"]), "fp64")),
+            "valu": profiled(level, args.riemann, args.slope, fp64["autotune"]["chosen_variant"], "fp64", "valu"),
             "counters_from": PROFILE_NOTE,
+            "kernel_sources_sha16": kernel_source_hash(),
             "reference_arithmetic": reference_arithmetic(nleaf / (g_ms * 1e-3), args.riemann, args.slope),
             "mixed": mixed_info,
             "variants": variants,
@@ -584,7 +595,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": PROFILED_TRAFFIC.get((level, args.riemann, args.slope, kernel_family(fp64["autotune"]["chosen_variant"]), "fp64")),
+            "traffic": profiled(level, args.riemann, args.slope, fp64["autotune"]["chosen_variant"], "fp64", "traffic"),
+            # What the counters show limits the kernel (the roofline above is
+            # the metric's HBM one).
+            "limiter": "VALU issue (SQ counters; see config.valu)",
         },
         "cpu_baseline": cpu,
     }

@@ -76,9 +76,12 @@ int rh_timers_reset(rh_ctx* ctx);
 /* Godunov kernel variant of a context: 0 = one thread per cell with the
  * octs of a workgroup strided across lanes; 1 = the eight cells of an oct in
  * adjacent lanes, sharing sibling states and fluxes by cross-lane shuffles,
- * 16 octs per workgroup; 2 (default) = the same with 8 octs per workgroup.
- * All produce identical results. */
-#define RH_KERNEL_VARIANTS 3
+ * 16 octs per workgroup; 2 (default) = the same with 8 octs per workgroup;
+ * 3, 4, 5 = bricks of 16, 32, 64 consecutive octs per workgroup that also
+ * share face states and fluxes between neighbouring octs through LDS (a
+ * 2x2x4, 4x4x2, 4x4x4 brick of octs on a uniform level).  All produce
+ * identical results. */
+#define RH_KERNEL_VARIANTS 6
 int rh_set_kernel_variant(rh_ctx* ctx, int variant);
 int rh_get_kernel_variant(rh_ctx* ctx, int* variant);
 

@@ -449,12 +449,27 @@ template <typename TC> RH_DEV void narrow(const double a[5], TC b[5]) {
     for (int i = 0; i < 5; ++i) b[i] = (TC)a[i];
 }
 
+// Primitive state of a cell for the slopes, traces and Riemann problems:
+// FP64 ctoprim in the FP64 form; in the mixed form the conservative state is
+// narrowed to FP32 and converted there.
+template <typename TC>
+RH_DEV void face_prim(const double u[5], TC q[5], const Gas<double>& gd, const Gas<TC>& gc) {
+    if constexpr (sizeof(TC) == 8) {
+        ctoprim(u, q, gd);
+    } else {
+        TC uc[5];
+        narrow(u, uc);
+        ctoprim(uc, q, gc);
+    }
+}
+
 // One face's Riemann problem: the FP64 form calls the bit-faithful solver on
-// double states; the mixed form solves the wave structure in FP32 and
-// assembles the flux in FP64.  ql/qr are rotated; F comes back rotated.
+// double states; the mixed form solves it in FP32 (its flux is widened to
+// FP64 where it enters the update or a reflux).  ql/qr are rotated; F comes
+// back rotated.
 template <int RIEMANN, bool MIXED, typename TC>
-RH_DEV void solve_face(const TC ql[5], const TC qr[5], double F[5], const Gas<TC>& gc, const Gas<double>& gd) {
-    if constexpr (MIXED) riemann_mixed<RIEMANN>(ql, qr, F, gc, gd);
+RH_DEV void solve_face(const TC ql[5], const TC qr[5], TC F[5], const Gas<TC>& gc, const Gas<double>& gd) {
+    if constexpr (MIXED) riemann_f<RIEMANN>(ql, qr, F, gc);
     else riemann<RIEMANN>(ql, qr, F, gd);
 }
 
@@ -590,9 +605,7 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
     TC qC[5];
     load_u(m.uold, m.ncell, cC, uC);
     {
-        double q[5];
-        ctoprim(uC, q, gd);
-        narrow(q, qC);
+        face_prim(uC, qC, gd, gc);
     }
 
     double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -613,16 +626,14 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
         n.ic0 = 0;
         return n;
     };
-    // Primitive state of a cell as read by the slope / ghost stencils (FP64
-    // ctoprim, then the compute type): a boundary index falls back to
-    // `fallback` (the reference copies the centre cell's state there;
-    // bound_type is never filled).
+    // Primitive state of a cell as read by the slope / ghost stencils
+    // (face_prim): a boundary index falls back to `fallback` (the reference
+    // copies the centre cell's state there; bound_type is never filled).
     auto prim = [&](int icell, const TC fallback[5], TC q[5]) {
         if (icell > 0) {
-            double u[5], qd[5];
+            double u[5];
             load_u(m.uold, m.ncell, icell, u);
-            ctoprim(u, qd, gd);
-            narrow(qd, q);
+            face_prim(u, q, gd, gc);
         } else {
             copy5(fallback, q);
         }
@@ -630,14 +641,13 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
     // Boundary face: own state with the normal momentum flipped, zero slopes
     // (HydroSolver.cpp:130-145).
     auto boundary = [&](int d, TC qm[5], TC qp[5]) {
-        double u[5], qd[5];
+        double u[5];
         load_u(m.uold, m.ncell, cC, u);
 #pragma unroll
         for (int i = 1; i <= 3; ++i)
             if (i == 1 + d) u[i] = u[i] * -1.0;
-        ctoprim(u, qd, gd);
         TC q[5];
-        narrow(qd, q);
+        face_prim(u, q, gd, gc);
         rotate(q, d);
         TC dq[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
         trace(q, dq, dtc, dxc, qm, qp, gc);
@@ -664,7 +674,7 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
 
         // ---- minus face (side 0): ql = ghost qm of L1, qr = own qp
         {
-            double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            TC F[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
             long long K = -1;
             if (L1.pos >= 0) {
                 if ((lv.leaf[L1.pos] >> L1.ic0) & 1) {
@@ -677,7 +687,7 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
                     rotate(F, d);
                     // L1's own plus-face flux into this cell (reflux).
                     TC Om[5], Op[5];
-                    double Fin[5];
+                    TC Fin[5];
                     own_trace(qL2, qL1, qC, d, slope_type, dtc, len, Om, Op, gc);
                     solve_face<RIEMANN, MIXED>(Om, GCp, Fin, gc, gd);
                     rotate(Fin, d);
@@ -713,7 +723,7 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
         }
         // ---- plus face (side 1): ql = own qm, qr = ghost qp of R1
         {
-            double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            TC F[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
             long long K = -1;
             if (R1.pos >= 0) {
                 if ((lv.leaf[R1.pos] >> R1.ic0) & 1) {
@@ -726,7 +736,7 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
                     rotate(F, d);
                     // R1's own minus-face flux into this cell (reflux).
                     TC Om[5], Op[5];
-                    double Fin[5];
+                    TC Fin[5];
                     own_trace(qC, qR1, qR2, d, slope_type, dtc, len, Om, Op, gc);
                     solve_face<RIEMANN, MIXED>(GCm, Op, Fin, gc, gd);
                     rotate(Fin, d);
@@ -765,6 +775,76 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
     ordered_apply<G_BLOCK>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t);
 }
 
+// The ordered accumulation of ordered_apply with the six incoming face
+// fluxes in LDS (slot f = face f of the receiving cell: the neighbour's own
+// flux on the shared face, in the compute type of the form) and their keys in
+// registers (negative: none).  Slot f's contribution is the reference's
+// ((dtdx * sign) * F) * 0.125 with the neighbour's face sign: -1 for a
+// neighbour on the minus side (its plus face), +1 on the plus side.
+template <int NT, typename TF>
+RH_DEV void ordered_apply_flux(const MeshDev& m, int cC, long long own_key, const double fs[5], double dtdx,
+                               const GasParams& gp, const TF (*s_f)[NT], const long long kin[6], int t,
+                               const double* unew_pre, double* u_out) {
+    const long long KOWN = own_key * 8 + 6;
+    long long ks[6];
+    int ix[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        ks[f] = kin[f] < 0 ? KMAX : kin[f];
+        ix[f] = f;
+    }
+    sort6(ks, ix);
+    double rs[6][5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double sdt = dtdx * ((ix[k] & 1) ? 1.0 : -1.0);
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) rs[k][iv] = sdt * (double)s_f[ix[k] * 5 + iv][t] * 0.125;
+    }
+    double u[5];
+    copy5(unew_pre, u);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const bool take = ks[k] < KOWN;
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            const double nv = u[iv] - rs[k][iv];
+            u[iv] = take ? nv : u[iv];
+        }
+    }
+    own_update(u, fs, dtdx, gp);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const bool take = ks[k] > KOWN && ks[k] != KMAX;
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            const double nv = u[iv] - rs[k][iv];
+            u[iv] = take ? nv : u[iv];
+        }
+    }
+    const size_t ci = (size_t)cC - 1;
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + ci] = u[iv];
+    if (u_out) copy5(u, u_out);
+}
+
+// Keys of the six incoming contributions of cell ic0 of oct p, from the
+// tables (face slot f; -1: none): the sibling's when it is a leaf, X's when
+// its oct is a same-level neighbour and X a leaf.
+RH_DEV void incoming_keys(const LevelDev& lv, int p, int ic0, int lm, long long kin[6]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const bool par = (ic0 >> d) & 1;
+        const int icS = ic0 ^ (1 << d);
+        const long long kS = ((lm >> icS) & 1) ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
+        const int E = lv.nbo[(size_t)(2 * d + (par ? 1 : 0)) * lv.noct + p];
+        long long kX = -1;
+        if (E >= 0 && ((lv.leaf[E] >> icS) & 1)) kX = ((long long)E * 8 + icS) * 8 + (par ? 2 * d : 2 * d + 1);
+        kin[2 * d] = par ? kS : kX;
+        kin[2 * d + 1] = par ? kX : kS;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Oct-in-lanes variant: the eight cells of an oct sit in eight adjacent lanes.
 // Per dimension each cell has one neighbour inside its oct (the sibling, lane
@@ -800,11 +880,11 @@ RH_DEV int xcd_block(int x, int n) {
 // variant 2); rh_autotune times both.
 template <int OCTS> constexpr int oct_block() { return OCTS * 8; }
 
-// Occupancy: both forms run two waves per SIMD (at most 256 VGPRs), with
-// 2.25 KB of LDS per oct (36 KB per 16-oct workgroup) for the incoming
-// contributions and their keys.  The registers above what the arithmetic
-// needs hold the next dimension's neighbour states, gathered ahead of their
-// use.
+// Occupancy: the FP64 form runs two waves per SIMD (at most 256 VGPRs), the
+// mixed form three (at most 168).  LDS per oct: the incoming face fluxes in
+// the compute type, 1.9 KB in the FP64 form and 0.9 KB in the mixed form.
+// The registers above what the arithmetic needs hold the next dimension's
+// neighbour states, gathered ahead of their use.
 //
 // Fused step (a level whose cells are all leaves and whose neighbours are
 // all at the same level, stepped with uold and unew exchanged between
@@ -815,15 +895,15 @@ template <int OCTS> constexpr int oct_block() { return OCTS * 8; }
 // ensemble, into its box's key in ckeys as k_courant_boxes does.  Otherwise
 // pre is unew and cpart and ckeys are null.
 template <int RIEMANN, bool MIXED, bool EXACT, int OCTS>
-__global__ __launch_bounds__(OCTS * 8) __attribute__((amdgpu_waves_per_eu(2, 8)))
+__global__ __launch_bounds__(OCTS * 8) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2, 8)))
 void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                                                           double dx, double inv_dx, const double* pre,
                                                           double* cpart, unsigned long long* ckeys,
                                                           double courant_factor) {
     using TC = typename std::conditional<MIXED, float, double>::type;
     constexpr int NT = oct_block<OCTS>();
-    __shared__ double s_r[30][NT];
-    __shared__ long long s_k[6][NT];
+    // The six incoming face fluxes of each cell (slot f*5 + iv).
+    __shared__ TC s_f[30][NT];
     const int t = threadIdx.x;
     const int ic0 = t & 7;
     const int praw = xcd_block(blockIdx.x, gridDim.x) * OCTS + (t >> 3);
@@ -849,26 +929,21 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
     TC qC[5];
     load_u(m.uold, m.ncell, cC, uC);
     {
-        double q[5];
-        ctoprim(uC, q, gd);
-        narrow(q, qC);
+        face_prim(uC, qC, gd, gc);
     }
     double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 
     auto prim = [&](int icell, const TC fallback[5], TC q[5]) {
         if (icell > 0) {
-            double u[5], qd[5];
+            double u[5];
             load_u(m.uold, m.ncell, icell, u);
-            ctoprim(u, qd, gd);
-            narrow(qd, q);
+            face_prim(u, q, gd, gc);
         } else {
             copy5(fallback, q);
         }
     };
     auto to_prim = [&](const double u[5], TC q[5]) {
-        double qd[5];
-        ctoprim(u, qd, gd);
-        narrow(qd, q);
+        face_prim(u, q, gd, gc);
     };
 
     // Across the oct face along dimension d: the neighbour oct E (or a
@@ -946,7 +1021,7 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
 
         // Internal face: this cell's own flux on the sibling side.
         const bool sib_leaf = (lm >> icS) & 1;
-        double Fint[5];
+        TC Fint[5];
         {
             TC ql[5], qr[5];
             sel5(par, GSm, OCm, ql);
@@ -954,15 +1029,14 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
             solve_face<RIEMANN, MIXED>(ql, qr, Fint, gc, gd);
             rotate(Fint, d);
 #pragma unroll
-            for (int iv = 0; iv < 5; ++iv) Fint[iv] = sib_leaf ? Fint[iv] : 0.0;
+            for (int iv = 0; iv < 5; ++iv) Fint[iv] = sib_leaf ? Fint[iv] : TC(0);
         }
-        double Fsib[5];
+        TC Fsib[5];
         shfl5(Fint, Fsib, b);  // the sibling's own flux on this cell's side
 
         // External face (across the oct boundary).
-        double Fext[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        double Fin[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        long long Kin = -1;
+        TC Fext[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+        TC Fin[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
         if (E >= 0) {
             if (cur.X2cell > 0) {
                 TC qX2[5];
@@ -982,20 +1056,18 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                 sel5(par, OXp, GCp, qr);
                 solve_face<RIEMANN, MIXED>(ql, qr, Fin, gc, gd);
                 rotate(Fin, d);
-                Kin = ((long long)E * 8 + icS) * 8 + (par ? 2 * d : 2 * d + 1);
             }
         } else if (rc.w & REC_REFINED_DEV) {
         } else if (rc.w & REC_BOUNDARY_DEV) {
             // The cell's own state, read again rather than kept in registers
             // through the loop for this rare branch.
-            double u[5], qd[5];
+            double u[5];
             load_u(m.uold, m.ncell, cC, u);
 #pragma unroll
             for (int i = 1; i <= 3; ++i)
                 if (i == 1 + d) u[i] = u[i] * -1.0;
-            ctoprim(u, qd, gd);
             TC qb[5];
-            narrow(qd, qb);
+            face_prim(u, qb, gd, gc);
             rotate(qb, d);
             TC dq0[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
             TC Bm[5], Bp[5];
@@ -1027,34 +1099,32 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         // Own fluxes in face order (minus face, then plus face).
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
-            const double Fm = par ? Fint[iv] : Fext[iv];
-            const double Fp = par ? Fext[iv] : Fint[iv];
+            const double Fm = (double)(par ? Fint[iv] : Fext[iv]);
+            const double Fp = (double)(par ? Fext[iv] : Fint[iv]);
             fs[iv] = fs[iv] + 1.0 * Fm;
             fs[iv] = fs[iv] + -1.0 * Fp;
         }
-        // Incoming contributions: from the sibling (its face toward this
-        // cell: plus face when it is on the minus side) and from X.
+        // Incoming fluxes: the sibling's own flux on the shared face (its plus
+        // face when it is on the minus side) and X's.
         const int slotS = 2 * d + (par ? 0 : 1);
         const int slotX = 2 * d + (par ? 1 : 0);
-        const double sgS = par ? -1.0 : 1.0;
-        const double sgX = par ? 1.0 : -1.0;
 #pragma unroll
         for (int iv = 0; iv < 5; ++iv) {
-            s_r[5 * slotS + iv][t] = dtdx * sgS * Fsib[iv] * 0.125;
-            s_r[5 * slotX + iv][t] = dtdx * sgX * Fin[iv] * 0.125;
+            s_f[5 * slotS + iv][t] = Fsib[iv];
+            s_f[5 * slotX + iv][t] = Fin[iv];
         }
-        s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
-        s_k[slotX][t] = Kin;
     }
+    long long kin[6];
+    incoming_keys(lv, p, ic0, lm, kin);
     if (!cpart && !ckeys) {
         if (!leaf) return;
-        ordered_apply<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn);
+        ordered_apply_flux<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, uXn, nullptr);
         return;
     }
     double cand = __builtin_inf();
     if (leaf) {
         double un[5];
-        ordered_apply<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_r, s_k, t, uXn, un);
+        ordered_apply_flux<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, uXn, un);
         cand = courant_cand_u(m, un, cC, dx, gp.gamma, courant_factor);
     }
     if (cpart) {
@@ -1073,6 +1143,404 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         if (t == 0) {
             double v = s_bmin[0];
             for (int w = 1; w < NT / 64; ++w) v = min_skip_nan(v, s_bmin[w]);
+            atomicMin(&ckeys[b0], dt_key(v));
+        }
+    } else if (cand != __builtin_inf()) {
+        atomicMin(&ckeys[lv.obox[p]], dt_key(cand));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Brick variant: OCTS consecutive octs of the list per workgroup, the cells
+// of an oct in eight adjacent lanes as above.  On a uniform level, whose
+// list is the Morton order of the octs, 64 / 32 / 16 consecutive octs are a
+// 4x4x4 / 4x4x2 / 2x2x4 brick of octs, so most external face neighbours X
+// of a cell are cells of the same workgroup.  Per dimension:
+//  A. every lane computes its cell's own and ghost face states (2 traces),
+//     sends the ghost state its sibling needs by shuffle and puts the one
+//     its external neighbour needs in LDS (s_g); a leaf cell whose X is a
+//     same-level leaf outside the workgroup registers a halo item;
+//  B. the halo items, compacted into the first lanes of the workgroup,
+//     compute what the neighbouring workgroup's lane would have: X's own and
+//     ghost face states and X's flux on the shared face (the contribution to
+//     the cell), and leave X's ghost state in the cell's s_g entry;
+//  C. every lane solves its two own faces (2 Riemann problems) with the
+//     neighbours' ghost states, shuffles the internal flux to its sibling
+//     and writes its external flux, as a contribution, into X's LDS slot
+//     when X is in the workgroup.
+// Per cell and dimension that is 2 traces and 2 solves, plus the halo items'
+// 2 traces and 1 solve for the brick's boundary cells (a quarter of the cells
+// of a 4x4x4 brick), against 4 traces and 3 solves in k_godunov_oct.  The
+// same operations on the same operands give the same values, so the results
+// are identical to the other variants for any level order; only the share of
+// halo items depends on it.  Contributions are kept per face slot in LDS and
+// applied in processing order as in ordered_apply, with the keys recomputed
+// from the tables at the end instead of stored.
+
+// Position of the k-th set bit (0-based) of a 64-bit mask that has more
+// than k set bits.
+RH_DEV int nth_set_bit(unsigned long long mask, int k) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const unsigned long long low = mask & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        const bool up = k >= c;
+        k = up ? k - c : k;
+        mask = up ? (mask >> w) : low;
+        pos += up ? w : 0;
+    }
+    return pos;
+}
+
+template <int OCTS> constexpr int brick_block() { return OCTS * 8; }
+
+// Occupancy: two waves per SIMD (at most 256 VGPRs).  LDS per lane: six
+// incoming face fluxes and one ghost state in the compute type (280 B in the
+// FP64 form, 140 B in the mixed form), plus 16 B of halo cell ids: 148 KB for
+// a 64-oct workgroup in the FP64 form, so one such workgroup fills a CU; 32-
+// and 16-oct workgroups let two or four share one.
+// `pre`, cpart, ckeys and courant_factor as in k_godunov_oct.
+template <int RIEMANN, bool MIXED, bool EXACT, int OCTS>
+__global__ __launch_bounds__(OCTS * 8) __attribute__((amdgpu_waves_per_eu(2, 8)))
+void k_godunov_brick(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt, double dx, double inv_dx,
+                     const double* pre, double* cpart, unsigned long long* ckeys, double courant_factor) {
+    using TC = typename std::conditional<MIXED, float, double>::type;
+    constexpr int NT = brick_block<OCTS>();
+    constexpr int NW = NT / 64;
+    __shared__ TC s_f[30][NT];  // the six incoming face fluxes of each cell
+    __shared__ TC s_g[5][NT];
+    // Halo registration of a dimension (ballot masks per wave, by dimension
+    // parity) and, per registering lane, the cells its halo item reads: the
+    // cell, X and X2.
+    __shared__ unsigned long long s_bal[2][NW];
+    __shared__ int4 s_hc[NT];
+    const int t = threadIdx.x;
+    const int ic0 = t & 7;
+    const int p0 = xcd_block(blockIdx.x, gridDim.x) * OCTS;
+    const int praw = p0 + (t >> 3);
+    const int noct = lv.noct;
+    const int pend = min(p0 + OCTS, noct);  // octs [p0, pend) are this workgroup's
+    // Lanes past the end of the list work on the last oct (valid memory) and
+    // write nothing, so every lane of an oct group takes part in the shuffles.
+    const bool valid = praw < noct;
+    const int p = valid ? praw : noct - 1;
+    const int lm = lv.leaf[p];
+    const bool leaf = valid && ((lm >> ic0) & 1);
+
+    const Gas<double> gd{gp.gamma, gp.gm1, gp.smalle, gp.smallp, 1.0 / gp.gm1};
+    const Gas<TC> gc{(TC)gp.gamma, (TC)gp.gm1, (TC)gp.smalle, (TC)gp.smallp, (TC)(1.0 / gp.gm1)};
+    const int slope_type = gp.slope_type, interpol_type = gp.interpol_type;
+    double dt;
+    step_of(lv, p, d_dt, dt, dx, inv_dx);
+    const double dtdx = dt / dx;
+    const TC dtc = (TC)dt, dxc = (TC)dx;
+    const Len<TC, EXACT> len{dxc, (TC)inv_dx};
+    const int cC = cell_id(m, lv.grid[p], ic0);
+
+    TC qC[5];
+    {
+        double u[5];
+        load_u(m.uold, m.ncell, cC, u);
+        face_prim(u, qC, gd, gc);
+    }
+    double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+
+    auto to_prim = [&](const double u[5], TC q[5]) {
+        face_prim(u, q, gd, gc);
+    };
+    auto prim = [&](int icell, const TC fallback[5], TC q[5]) {
+        if (icell > 0) {
+            double u[5];
+            load_u(m.uold, m.ncell, icell, u);
+            to_prim(u, q);
+        } else {
+            copy5(fallback, q);
+        }
+    };
+
+    // Across the oct face along dimension d, as in k_godunov_oct; the next
+    // dimension's X and X2 are gathered while the current one is computed.
+    struct Across {
+        int E;
+        int4 rc;
+        int Xcell, X2cell;
+    };
+    auto across = [&](int d) {
+        Across a;
+        const bool pd = (ic0 >> d) & 1;
+        const int icS = ic0 ^ (1 << d);
+        a.E = lv.nbo[(size_t)(2 * d + (pd ? 1 : 0)) * noct + p];
+        a.rc = make_int4(0, 0, 0, 0);
+        a.X2cell = 0;
+        if (a.E >= 0) {
+            const int g = lv.grid[a.E];
+            a.Xcell = cell_id(m, g, icS);
+            if ((lv.leaf[a.E] >> icS) & 1) a.X2cell = cell_id(m, g, ic0);
+        } else {
+            a.rc = lv.rec[-1 - a.E];
+            a.Xcell = a.rc.x;
+        }
+        return a;
+    };
+    Across nx = across(0);
+    double uXn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (nx.Xcell > 0) load_u(m.uold, m.ncell, nx.Xcell, uXn);
+
+    // A leaf cell whose X along d is a same-level leaf outside the workgroup
+    // registers a halo item for d.
+    auto register_halo = [&](const Across& a, int d) {
+        const bool halo = leaf && a.E >= 0 && a.X2cell > 0 && !(a.E >= p0 && a.E < pend);
+        const unsigned long long bal = __ballot(halo);
+        if ((t & 63) == 0) s_bal[d & 1][t >> 6] = bal;
+        s_hc[t] = make_int4(cC, a.Xcell, a.X2cell, 0);
+    };
+    auto halo_count = [&](int d) {
+        int n = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) n += __popcll(s_bal[d & 1][w]);
+        return n;
+    };
+    // Lane of the h-th registered cell of dimension d (in lane order).
+    auto halo_lane = [&](int d, int h) {
+        int w = 0, k = h;
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i) {
+            const int c = __popcll(s_bal[d & 1][w]);
+            const bool on = k >= c;
+            k = on ? k - c : k;
+            w = on ? w + 1 : w;
+        }
+        return w * 64 + nth_set_bit(s_bal[d & 1][w], k);
+    };
+    // Lane t takes halo item t of dimension d: its cell's lane, and the
+    // states of the cell, X and X2 gathered one dimension ahead of their
+    // use (a lane without an item loads its own cell, a valid address).
+    int h_tc = t;
+    double hC[5], hX[5], hX2[5];
+    auto halo_gather = [&](int d) {
+        const bool has = t < halo_count(d);
+        h_tc = has ? halo_lane(d, t) : t;
+        const int4 ids = has ? s_hc[h_tc] : make_int4(cC, cC, cC, 0);
+        load_u(m.uold, m.ncell, ids.x, hC);
+        load_u(m.uold, m.ncell, ids.y, hX);
+        load_u(m.uold, m.ncell, ids.z, hX2);
+    };
+    register_halo(nx, 0);
+    __syncthreads();
+    halo_gather(0);
+
+    constexpr int DIM_UNROLL = (MIXED || RIEMANN != 2) ? 3 : 1;
+#pragma unroll DIM_UNROLL
+    for (int d = 0; d < 3; ++d) {
+        const int b = 1 << d;
+        const bool par = (ic0 >> d) & 1;  // 0: sibling on the plus side, X on the minus side
+        const int icS = ic0 ^ b;
+        const Across cur = nx;
+        double uX[5];
+        copy5(uXn, uX);
+        if (d < 2) {
+            nx = across(d + 1);
+            if (nx.Xcell > 0) load_u(m.uold, m.ncell, nx.Xcell, uXn);
+        } else {
+            // Last dimension: the cell's own unew entry before this update
+            // (no other lane writes it in this kernel).
+            load_u(pre, m.ncell, cC, uXn);
+        }
+        const int E = cur.E;
+        const bool xleaf = E >= 0 && cur.X2cell > 0;     // X is a same-level leaf
+        const bool xin = E >= p0 && E < pend;            // X's oct is in this workgroup
+        const int lane_x = (E - p0) * 8 + icS;           // X's lane when xin
+
+        // A. Own and ghost face states of the cell.
+        TC OCm[5], OCp[5], Gsib[5];
+        {
+            TC qS[5], qX[5];
+            shfl5(qC, qS, b);
+            if (cur.Xcell > 0) to_prim(uX, qX);
+            else copy5(qC, qX);
+            TC qL1[5], qR1[5];
+            sel5(par, qS, qX, qL1);
+            sel5(par, qX, qS, qR1);
+            own_trace(qL1, qC, qR1, d, slope_type, dtc, len, OCm, OCp, gc);
+            TC GCm[5], GCp[5];
+            ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, len, GCm, GCp, gc);
+            // Toward the sibling: its left neighbour's qm / right neighbour's
+            // qp; toward X the other one.
+            TC Gown[5];
+            sel5(par, GCp, GCm, Gown);
+            shfl5(Gown, Gsib, b);
+            TC Gext[5];
+            sel5(par, GCm, GCp, Gext);
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) s_g[iv][t] = Gext[iv];
+        }
+        if (d < 2) register_halo(nx, d + 1);
+        __syncthreads();
+
+        // B. Halo items: lane h takes the h-th registered cell (in lane
+        // order), whose states it gathered during the previous dimension.
+        if (t < halo_count(d)) {
+            const int tc = h_tc;
+            const int pc = p0 + (tc >> 3), icc = tc & 7;
+            // The cell's time step and width (its box's, in an ensemble).
+            double dth, dxh = dx, ivh = inv_dx;
+            step_of(lv, pc, d_dt, dth, dxh, ivh);
+            const TC dtch = (TC)dth;
+            const Len<TC, EXACT> lenh{(TC)dxh, (TC)ivh};
+            const bool parc = (icc >> d) & 1;
+            TC qc[5], qx[5], qx2[5];
+            to_prim(hC, qc);
+            to_prim(hX, qx);
+            to_prim(hX2, qx2);
+            TC a5[5], c5[5];
+            sel5(parc, qc, qx2, a5);
+            sel5(parc, qx2, qc, c5);
+            TC GXm[5], GXp[5], OXm[5], OXp[5];
+            ghost_trace(a5, qx, c5, d, interpol_type, dtch, lenh, GXm, GXp, gc);
+            own_trace(a5, qx, c5, d, slope_type, dtch, lenh, OXm, OXp, gc);
+            TC Gc[5];
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) Gc[iv] = s_g[iv][tc];
+            // X's own flux on the shared face: its side 0 when X is on the
+            // cell's plus side, else its side 1.
+            TC Fin[5];
+            {
+                TC ql[5], qr[5];
+                sel5(parc, Gc, OXm, ql);
+                sel5(parc, OXp, Gc, qr);
+                solve_face<RIEMANN, MIXED>(ql, qr, Fin, gc, gd);
+                rotate(Fin, d);
+            }
+            const int slotX = 2 * d + (parc ? 1 : 0);
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) s_f[5 * slotX + iv][tc] = Fin[iv];
+            // The cell's s_g entry now holds X's ghost state toward the cell
+            // (no other lane reads it: X is outside the workgroup).
+            TC GXc[5];
+            sel5(parc, GXp, GXm, GXc);
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) s_g[iv][tc] = GXc[iv];
+        }
+        if (d < 2) halo_gather(d + 1);
+        __syncthreads();
+
+        // C. The cell's two own faces.
+        const bool sib_leaf = (lm >> icS) & 1;
+        TC Fint[5];
+        {
+            TC ql[5], qr[5];
+            sel5(par, Gsib, OCm, ql);
+            sel5(par, OCp, Gsib, qr);
+            solve_face<RIEMANN, MIXED>(ql, qr, Fint, gc, gd);
+            rotate(Fint, d);
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) Fint[iv] = sib_leaf ? Fint[iv] : TC(0);
+        }
+        TC Fsib[5];
+        shfl5(Fint, Fsib, b);  // the sibling's own flux on this cell's side
+
+        TC Fext[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+        const int4 rc = cur.rc;
+        if (E >= 0) {
+            if (xleaf) {
+                const int src = xin ? lane_x : t;
+                TC GX[5];
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) GX[iv] = s_g[iv][src];
+                TC ql[5], qr[5];
+                sel5(par, OCm, GX, ql);
+                sel5(par, GX, OCp, qr);
+                solve_face<RIEMANN, MIXED>(ql, qr, Fext, gc, gd);
+                rotate(Fext, d);
+                if (xin && leaf) {
+                    // This cell's flux is X's incoming flux on this face.
+                    const int slotXX = 2 * d + (par ? 0 : 1);
+#pragma unroll
+                    for (int iv = 0; iv < 5; ++iv) s_f[5 * slotXX + iv][lane_x] = Fext[iv];
+                }
+            }
+        } else if (rc.w & REC_REFINED_DEV) {
+        } else if (rc.w & REC_BOUNDARY_DEV) {
+            double u[5];
+            load_u(m.uold, m.ncell, cC, u);
+#pragma unroll
+            for (int i = 1; i <= 3; ++i)
+                if (i == 1 + d) u[i] = u[i] * -1.0;
+            TC qb[5];
+            face_prim(u, qb, gd, gc);
+            rotate(qb, d);
+            TC dq0[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+            TC Bm[5], Bp[5];
+            trace(qb, dq0, dtc, dxc, Bm, Bp, gc);
+            TC ql[5], qr[5];
+            sel5(par, OCm, Bm, ql);
+            sel5(par, Bp, OCp, qr);
+            solve_face<RIEMANN, MIXED>(ql, qr, Fext, gc, gd);
+            rotate(Fext, d);
+        } else {
+            TC qX[5], qnl[5], qnr[5];
+            prim(rc.x, qC, qX);
+            prim(rc.y, qX, qnl);
+            prim(rc.z, qX, qnr);
+            TC Gm[5], Gp[5];
+            ghost_trace(qnl, qX, qnr, d, interpol_type, dtc, len, Gm, Gp, gc);
+            TC ql[5], qr[5];
+            sel5(par, OCm, Gm, ql);
+            sel5(par, Gp, OCp, qr);
+            solve_face<RIEMANN, MIXED>(ql, qr, Fext, gc, gd);
+            rotate(Fext, d);
+            if (leaf) {
+                const int slot = lv.rec_slot[-1 - E] + face_slot(ic0, d);
+                const double sg = par ? -1.0 : 1.0;
+#pragma unroll
+                for (int iv = 0; iv < 5; ++iv) lv.slot_r[(size_t)slot * 5 + iv] = dtdx * sg * Fext[iv] * 0.125;
+            }
+        }
+
+        // Own fluxes in face order (minus face, then plus face).
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            const double Fm = (double)(par ? Fint[iv] : Fext[iv]);
+            const double Fp = (double)(par ? Fext[iv] : Fint[iv]);
+            fs[iv] = fs[iv] + 1.0 * Fm;
+            fs[iv] = fs[iv] + -1.0 * Fp;
+        }
+        // The sibling's own flux on the shared face.
+        const int slotS = 2 * d + (par ? 0 : 1);
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) s_f[5 * slotS + iv][t] = Fsib[iv];
+        // s_g and s_bal are rewritten by the next dimension; the last
+        // dimension's barrier also publishes every contribution slot.
+        __syncthreads();
+    }
+
+    long long kin[6];
+    incoming_keys(lv, p, ic0, lm, kin);
+    if (!cpart && !ckeys) {
+        if (!leaf) return;
+        ordered_apply_flux<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, uXn, nullptr);
+        return;
+    }
+    double cand = __builtin_inf();
+    if (leaf) {
+        double un[5];
+        ordered_apply_flux<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, uXn, un);
+        cand = courant_cand_u(m, un, cC, dx, gp.gamma, courant_factor);
+    }
+    if (cpart) {
+        block_min_n<NT>(cand, cpart);
+        return;
+    }
+    const int b0 = lv.obox[p0], b1 = lv.obox[pend - 1];
+    if (b0 == b1) {
+        __shared__ double s_bmin[NW];
+        cand = wave_min(cand);
+        if ((t & 63) == 0) s_bmin[t >> 6] = cand;
+        __syncthreads();
+        if (t == 0) {
+            double v = s_bmin[0];
+            for (int w = 1; w < NW; ++w) v = min_skip_nan(v, s_bmin[w]);
             atomicMin(&ckeys[b0], dt_key(v));
         }
     } else if (cand != __builtin_inf()) {
@@ -1557,7 +2025,9 @@ hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, dou
 }
 
 int godunov_blocks(const LevelDev& lv, int variant) {
-    return variant == 1 ? (lv.noct + 15) / 16 : variant == 2 ? (lv.noct + 7) / 8 : (lv.noct + G_OCTS - 1) / G_OCTS;
+    const int octs = variant == 1 ? 16 : variant == 2 ? 8 : variant == 3 ? 16 : variant == 4 ? 32 : variant == 5 ? 64
+                                                                                                           : G_OCTS;
+    return (lv.noct + octs - 1) / octs;
 }
 
 hipError_t launch_courant_final(const double* d_part, int nparts, double dt_init, double* d_dt, hipStream_t stream) {
@@ -1614,7 +2084,19 @@ template <int R, bool MIXED, bool EXACT>
 void launch_form(int variant, const MeshDev& m, const LevelDev& lv, const GasParams& gp, const double* d_dt,
                  double dx, double inv_dx, const FusedCourant& fc, hipStream_t stream) {
     const double* pre = fc.pre_from_uold ? m.uold : m.unew;
-    if (variant == 1)
+    if (variant >= 3) {
+        const int octs = variant == 3 ? 16 : variant == 4 ? 32 : 64;
+        const dim3 grid((lv.noct + octs - 1) / octs);
+        if (variant == 3)
+            hipLaunchKernelGGL((k_godunov_brick<R, MIXED, EXACT, 16>), grid, dim3(brick_block<16>()), 0, stream, m,
+                               lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.keys, fc.courant_factor);
+        else if (variant == 4)
+            hipLaunchKernelGGL((k_godunov_brick<R, MIXED, EXACT, 32>), grid, dim3(brick_block<32>()), 0, stream, m,
+                               lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.keys, fc.courant_factor);
+        else
+            hipLaunchKernelGGL((k_godunov_brick<R, MIXED, EXACT, 64>), grid, dim3(brick_block<64>()), 0, stream, m,
+                               lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.keys, fc.courant_factor);
+    } else if (variant == 1)
         hipLaunchKernelGGL((k_godunov_oct<R, MIXED, EXACT, 16>), dim3((lv.noct + 15) / 16), dim3(oct_block<16>()), 0,
                            stream, m, lv, gp, d_dt, dx, inv_dx, pre, fc.part, fc.keys, fc.courant_factor);
     else if (variant == 2)

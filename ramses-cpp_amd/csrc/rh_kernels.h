@@ -90,19 +90,21 @@ hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, dou
 // Godunov update of one level; dt is read from device memory (*d_dt).
 // variant 0: one thread per cell, octs strided across lanes; variants 1 and
 // 2: the eight cells of an oct in adjacent lanes, sharing by cross-lane
-// shuffles, 16 (variant 1) or 8 (variant 2) octs per workgroup.
+// shuffles, 16 (variant 1) or 8 (variant 2) octs per workgroup; variants 3,
+// 4, 5: bricks of 16, 32, 64 consecutive octs per workgroup that share face
+// states and fluxes between octs through LDS as well.
 // dt is d_dt[0] for a single mesh and d_dt[box] for an ensemble.  inv_dx is
 // 1/dx when dx is a power of two (the FP64 form then multiplies by it, with
 // identical results), else 0; an ensemble reads dx and 1/dx per box and
 // passes inv_dx != 0 when every box has an exact reciprocal.
-constexpr int KERNEL_VARIANTS = 3;
+constexpr int KERNEL_VARIANTS = 6;
 // Fused step of a single-level run or an ensemble: with pre_from_uold the
 // launch reads the cells' pre-update unew from uold (what set_unew copies
 // into unew), and each leaf cell also takes the Courant candidate of its new
 // state (what set_uold copies into uold) into a block minimum in
 // part[godunov_blocks(lv, variant)] (single mesh, part non-null) or into
 // its box's key in keys (ensemble, keys non-null; as launch_courant_boxes).
-// Only for variants 1 and 2 on levels with only leaf cells and same-level
+// Only for variants 1 to 5 on levels with only leaf cells and same-level
 // neighbours (no records, no reflux targets).
 struct FusedCourant {
     bool pre_from_uold = false;

@@ -76,7 +76,7 @@ template <typename T> struct Gas {
     T gm1;       // gamma - 1
     T smalle;    // ((1e-10*1e-10)/gamma)/(gamma-1), ctoprim floor
     T smallp;    // (1e-10*1e-10)/gamma, Riemann pressure floor factor
-    T inv_gm1;   // 1/(gamma-1), used only by the mixed form's FP64 assembly
+    T inv_gm1;   // 1/(gamma-1), used only by the mixed form's FP32 Riemann solve
 };
 
 // Conservative -> primitive (HydroSolver.cpp:613-651, nener=0).
@@ -467,26 +467,24 @@ RH_DEV void riemann(const T ql[5], const T qr[5], T flux[5], const Gas<T>& g) {
 }
 
 // ---------------------------------------------------------------------------
-// Mixed form: the wave structure (sound speeds, wave speeds, star states) is
-// solved in FP32 from FP32 face states; the conservative states, physical
-// fluxes and the flux assembly F_K + S_K (U*_K - U_K) are FP64.
+// Mixed form: the whole Riemann solve -- sound speeds, wave speeds, star
+// states, physical fluxes and the flux assembly -- is FP32 on FP32 face
+// states (hardware reciprocal and square root); the flux is widened to FP64
+// for the flux difference, the reflux and the update, which stay FP64.  The
+// branch structure follows the FP64 solvers above; the primitive <->
+// conservative conversions multiply by 1/(gamma-1).
 
-RH_DEV void widen(const float a[5], double b[5]) {
-#pragma unroll
-    for (int i = 0; i < 5; ++i) b[i] = (double)a[i];
-}
-
-RH_DEV void prim_to_cons_m(const double q[5], double u[5], const Gas<double>& g) {
+RH_DEV void prim_to_cons_f(const float q[5], float u[5], const Gas<float>& g) {
     u[0] = q[0];
     u[1] = q[0] * q[1];
     u[2] = q[0] * q[2];
     u[3] = q[0] * q[3];
-    u[4] = 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + q[4] * g.inv_gm1;
+    u[4] = 0.5f * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + q[4] * g.inv_gm1;
 }
 
-RH_DEV void phys_flux_m(const double q[5], double f[5], const Gas<double>& g) {
-    const double mu = q[0] * q[1];
-    const double E = 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + q[4] * g.inv_gm1;
+RH_DEV void phys_flux_f(const float q[5], float f[5], const Gas<float>& g) {
+    const float mu = q[0] * q[1];
+    const float E = 0.5f * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + q[4] * g.inv_gm1;
     f[0] = mu;
     f[1] = mu * q[1] + q[4];
     f[2] = mu * q[2];
@@ -494,8 +492,7 @@ RH_DEV void phys_flux_m(const double q[5], double f[5], const Gas<double>& g) {
     f[4] = (E + q[4]) * q[1];
 }
 
-RH_DEV void riemann_hllc_m(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
-                           const Gas<double>& gd) {
+RH_DEV void riemann_hllc_f(const float ql[5], const float qr[5], float flux[5], const Gas<float>& gf) {
     const float rl = smax(ql[0], 1e-10f), ul = ql[1], pl = smax(ql[4], rl * gf.smallp);
     const float rr = smax(qr[0], 1e-10f), ur = qr[1], pr = smax(qr[4], rr * gf.smallp);
     const float cl = ssqrt(cs2(rl, ql[4], gf));
@@ -503,28 +500,23 @@ RH_DEV void riemann_hllc_m(const float ql[5], const float qr[5], double flux[5],
     const float cmax = smax(cl, cr);
     const float sl = smin(ul, ur) - cmax;
     const float sr = smax(ul, ur) + cmax;
-    double qd[5];
     if (sl > 0.0f) {
-        widen(ql, qd);
-        phys_flux_m(qd, flux, gd);
+        phys_flux_f(ql, flux, gf);
         return;
     }
     if (sr < 0.0f) {
-        widen(qr, qd);
-        phys_flux_m(qd, flux, gd);
+        phys_flux_f(qr, flux, gf);
         return;
     }
     const float rcl = rl * (ul - sl);
     const float rcr = rr * (sr - ur);
     const float div = rcr + rcl;
     if (div < 1e-5f) {
-        double fl[5], fr[5];
-        widen(ql, qd);
-        phys_flux_m(qd, fl, gd);
-        widen(qr, qd);
-        phys_flux_m(qd, fr, gd);
+        float fl[5], fr[5];
+        phys_flux_f(ql, fl, gf);
+        phys_flux_f(qr, fr, gf);
 #pragma unroll
-        for (int i = 0; i < 5; ++i) flux[i] = 0.5 * (fl[i] + fr[i]);
+        for (int i = 0; i < 5; ++i) flux[i] = 0.5f * (fl[i] + fr[i]);
         return;
     }
     const float idiv = __builtin_amdgcn_rcpf(div);
@@ -532,120 +524,112 @@ RH_DEV void riemann_hllc_m(const float ql[5], const float qr[5], double flux[5],
     ustar = smax(-1e6f, smin(1e6f, ustar));
     float pstar = (rcr * pl + rcl * pr + rcl * rcr * (ul - ur)) * idiv;
     pstar = smax(pstar, 0.0f);
+    // One side of the contact, as in riemann_hllc.
     const bool left = ustar > 0.0f;
     const float* qk = left ? ql : qr;
     const float rk = left ? rl : rr, uk = left ? ul : ur, pk = left ? pl : pr, sk = left ? sl : sr;
     const float den = left ? smin(sl - ustar, -1e-10f) : smax(sr - ustar, 1e-10f);
-    const float rstar = rk * (sk - uk) * __builtin_amdgcn_rcpf(den);
-    double qkd[5], qs[5], fk[5], ukv[5], usv[5];
-    widen(qk, qkd);
+    const float iden = __builtin_amdgcn_rcpf(den);
+    const float rstar = rk * (sk - uk) * iden;
+    float qs[5], fk[5], ukv[5], usv[5];
     qs[0] = rstar;
     qs[1] = ustar;
-    qs[2] = qkd[2];
-    qs[3] = qkd[3];
-    qs[4] = (double)smax(pstar, rstar * gf.smallp);
-    phys_flux_m(qkd, fk, gd);
-    prim_to_cons_m(qkd, ukv, gd);
-    prim_to_cons_m(qs, usv, gd);
-    const double skd = sk, ukd = uk, dend = den;
-    usv[4] = ((skd - ukd) * ukv[4] - (double)pk * ukd + (double)pstar * (double)ustar) / dend;
+    qs[2] = qk[2];
+    qs[3] = qk[3];
+    qs[4] = smax(pstar, rstar * gf.smallp);
+    phys_flux_f(qk, fk, gf);
+    prim_to_cons_f(qk, ukv, gf);
+    prim_to_cons_f(qs, usv, gf);
+    usv[4] = ((sk - uk) * ukv[4] - pk * uk + pstar * ustar) * iden;
 #pragma unroll
-    for (int i = 0; i < 5; ++i) flux[i] = fk[i] + skd * (usv[i] - ukv[i]);
+    for (int i = 0; i < 5; ++i) flux[i] = fk[i] + sk * (usv[i] - ukv[i]);
 }
 
-RH_DEV void riemann_hll_m(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
-                          const Gas<double>& gd) {
+RH_DEV void riemann_hll_f(const float ql[5], const float qr[5], float flux[5], const Gas<float>& gf) {
     const float al = ssqrt(cs2(ql[0], ql[4], gf));
     const float ar = ssqrt(cs2(qr[0], qr[4], gf));
     const float sl = smin(0.0f, smin(ql[1] - al, qr[1] - ar));
     const float sr = smax(0.0f, smax(ql[1] + al, qr[1] + ar));
-    double a[5], b[5], fl[5], fr[5], ul[5], ur[5];
-    widen(ql, a);
-    widen(qr, b);
-    phys_flux_m(a, fl, gd);
-    phys_flux_m(b, fr, gd);
+    float fl[5], fr[5];
+    phys_flux_f(ql, fl, gf);
+    phys_flux_f(qr, fr, gf);
     if (sr - sl < 1e-20f) {
 #pragma unroll
-        for (int i = 0; i < 5; ++i) flux[i] = 0.5 * (fl[i] + fr[i]);
+        for (int i = 0; i < 5; ++i) flux[i] = 0.5f * (fl[i] + fr[i]);
         return;
     }
-    prim_to_cons_m(a, ul, gd);
-    prim_to_cons_m(b, ur, gd);
-    const double sld = sl, srd = sr;
-    const double inv = 1.0 / (srd - sld);
+    float ul[5], ur[5];
+    prim_to_cons_f(ql, ul, gf);
+    prim_to_cons_f(qr, ur, gf);
+    const float inv = __builtin_amdgcn_rcpf(sr - sl);
 #pragma unroll
-    for (int i = 0; i < 5; ++i) flux[i] = (srd * fl[i] - sld * fr[i] + srd * sld * (ur[i] - ul[i])) * inv;
+    for (int i = 0; i < 5; ++i) flux[i] = (sr * fl[i] - sl * fr[i] + sr * sl * (ur[i] - ul[i])) * inv;
 }
 
-RH_DEV void riemann_llf_m(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
-                          const Gas<double>& gd) {
+RH_DEV void riemann_llf_f(const float ql[5], const float qr[5], float flux[5], const Gas<float>& gf) {
     const float al = ssqrt(cs2(ql[0], ql[4], gf));
     const float ar = ssqrt(cs2(qr[0], qr[4], gf));
-    const double a_max = (double)smax(sabs(ql[1]) + al, sabs(qr[1]) + ar);
-    double a[5], b[5], fl[5], fr[5], ul[5], ur[5];
-    widen(ql, a);
-    widen(qr, b);
-    phys_flux_m(a, fl, gd);
-    phys_flux_m(b, fr, gd);
-    prim_to_cons_m(a, ul, gd);
-    prim_to_cons_m(b, ur, gd);
+    const float a_max = smax(sabs(ql[1]) + al, sabs(qr[1]) + ar);
+    float fl[5], fr[5], ul[5], ur[5];
+    phys_flux_f(ql, fl, gf);
+    phys_flux_f(qr, fr, gf);
+    prim_to_cons_f(ql, ul, gf);
+    prim_to_cons_f(qr, ur, gf);
 #pragma unroll
-    for (int i = 0; i < 5; ++i) flux[i] = 0.5 * (fl[i] + fr[i]) - 0.5 * a_max * (ur[i] - ul[i]);
+    for (int i = 0; i < 5; ++i) flux[i] = 0.5f * (fl[i] + fr[i]) - 0.5f * a_max * (ur[i] - ul[i]);
 }
 
-RH_DEV void riemann_acoustic_m(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
-                               const Gas<double>& gd) {
-    // The sampled Godunov state is found in FP32; its flux is assembled in FP64.
-    float qg[5];
-    {
-        const float rl = smax(ql[0], 1e-10f), ul = ql[1], pl = smax(ql[4], rl * gf.smallp);
-        const float rr = smax(qr[0], 1e-10f), ur = qr[1], pr = smax(qr[4], rr * gf.smallp);
-        const float cl = ssqrt(cs2(rl, ql[4], gf)), cr = ssqrt(cs2(rr, qr[4], gf));
-        const float wl = cl * rl, wr = cr * rr;
-        const float iw = __builtin_amdgcn_rcpf(wl + wr + 1e-20f);
-        const float pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) * iw;
-        const float ustar = ((wr * ur + wl * ul) + (pl - pr)) * iw;
-        const float sgnm = (ustar >= 0.0f) ? 1.0f : -1.0f;
-        const bool lft = sgnm == 1.0f;
-        const float ro = lft ? rl : rr, uo = lft ? ul : ur, po = lft ? pl : pr, co = lft ? cl : cr;
-        float rstar = smax(ro + dv(pstar - po, smax(co * co, 1e-20f)), 1e-10f);
-        float cstar = smax(ssqrt(smax(dv(gf.gamma * pstar, rstar), 1e-20f)), 1e-10f);
-        float spout = co - sgnm * uo;
-        float spin = cstar - sgnm * ustar;
-        float ushock = smax(0.5f * (spin + spout), -sgnm * ustar);
-        if (pstar >= po) {
-            spout = ushock;
-            spin = spout;
-        }
-        float rg, ug, pg;
-        if (spout < 0.0f) {
-            rg = ro; ug = uo; pg = po;
-        } else if (spin >= 0.0f) {
-            rg = rstar; ug = ustar; pg = pstar;
-        } else {
-            const float frac = dv(spout, spout - spin + 1e-20f);
-            rg = frac * rstar + (1.0f - frac) * ro;
-            ug = frac * ustar + (1.0f - frac) * uo;
-            pg = frac * pstar + (1.0f - frac) * po;
-        }
-        qg[0] = rg;
-        qg[1] = ug;
-        qg[2] = lft ? ql[2] : qr[2];
-        qg[3] = lft ? ql[3] : qr[3];
-        qg[4] = smax(pg, rg * gf.smallp);
+RH_DEV void riemann_acoustic_f(const float ql[5], const float qr[5], float flux[5], const Gas<float>& gf) {
+    const float rl = smax(ql[0], 1e-10f), ul = ql[1], pl = smax(ql[4], rl * gf.smallp);
+    const float rr = smax(qr[0], 1e-10f), ur = qr[1], pr = smax(qr[4], rr * gf.smallp);
+    const float cl = ssqrt(cs2(rl, ql[4], gf)), cr = ssqrt(cs2(rr, qr[4], gf));
+    const float wl = cl * rl, wr = cr * rr;
+    const float iw = __builtin_amdgcn_rcpf(wl + wr + 1e-20f);
+    const float pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) * iw;
+    const float ustar = ((wr * ur + wl * ul) + (pl - pr)) * iw;
+    const float sgnm = (ustar >= 0.0f) ? 1.0f : -1.0f;
+    const bool lft = sgnm == 1.0f;
+    const float ro = lft ? rl : rr, uo = lft ? ul : ur, po = lft ? pl : pr, co = lft ? cl : cr;
+    float rstar = smax(ro + dv(pstar - po, smax(co * co, 1e-20f)), 1e-10f);
+    float cstar = smax(ssqrt(smax(dv(gf.gamma * pstar, rstar), 1e-20f)), 1e-10f);
+    float spout = co - sgnm * uo;
+    float spin = cstar - sgnm * ustar;
+    float ushock = smax(0.5f * (spin + spout), -sgnm * ustar);
+    if (pstar >= po) {
+        spout = ushock;
+        spin = spout;
     }
-    double q[5];
-    widen(qg, q);
-    phys_flux_m(q, flux, gd);
+    float rg, ug, pg;
+    if (spout < 0.0f) {
+        rg = ro;
+        ug = uo;
+        pg = po;
+    } else if (spin >= 0.0f) {
+        rg = rstar;
+        ug = ustar;
+        pg = pstar;
+    } else {
+        const float frac = dv(spout, spout - spin + 1e-20f);
+        rg = frac * rstar + (1.0f - frac) * ro;
+        ug = frac * ustar + (1.0f - frac) * uo;
+        pg = frac * pstar + (1.0f - frac) * po;
+    }
+    float qg[5];
+    qg[0] = rg;
+    qg[1] = ug;
+    qg[2] = lft ? ql[2] : qr[2];
+    qg[3] = lft ? ql[3] : qr[3];
+    qg[4] = smax(pg, rg * gf.smallp);
+    phys_flux_f(qg, flux, gf);
 }
 
+// The mixed form's face flux (FP32).
 template <int RIEMANN>
-RH_DEV void riemann_mixed(const float ql[5], const float qr[5], double flux[5], const Gas<float>& gf,
-                          const Gas<double>& gd) {
-    if constexpr (RIEMANN == 2) riemann_hllc_m(ql, qr, flux, gf, gd);
-    else if constexpr (RIEMANN == 1) riemann_hll_m(ql, qr, flux, gf, gd);
-    else if constexpr (RIEMANN == 4) riemann_acoustic_m(ql, qr, flux, gf, gd);
-    else riemann_llf_m(ql, qr, flux, gf, gd);
+RH_DEV void riemann_f(const float ql[5], const float qr[5], float flux[5], const Gas<float>& gf) {
+    if constexpr (RIEMANN == 2) riemann_hllc_f(ql, qr, flux, gf);
+    else if constexpr (RIEMANN == 1) riemann_hll_f(ql, qr, flux, gf);
+    else if constexpr (RIEMANN == 4) riemann_acoustic_f(ql, qr, flux, gf);
+    else riemann_llf_f(ql, qr, flux, gf);
 }
 
 }  // namespace rh

@@ -22,7 +22,7 @@ RH_HOST_COHERENT, RH_DEVICE_RESIDENT = 0, 1
 RH_SYNC_TO_HOST, RH_SYNC_TO_DEVICE = 1, 2
 RH_UOLD, RH_UNEW = 1, 2
 TK_GODUNOV, TK_COURANT, TK_SET_UNEW, TK_SET_UOLD, TK_STEP_JOIN = range(5)
-RH_KERNEL_VARIANTS = 3  # 0 per-cell; 1, 2 oct-in-lanes with 16 / 8 octs per workgroup
+RH_KERNEL_VARIANTS = 6  # 0 per-cell; 1, 2 oct-in-lanes with 16 / 8 octs per workgroup; 3, 4, 5 bricks of 16 / 32 / 64 octs
 
 _IP = C.POINTER(C.c_int)
 _DP = C.POINTER(C.c_double)
@@ -382,20 +382,27 @@ class EnsembleBox:
     nx: int
     level: int
     dx: float
-    normal: np.ndarray
-    left: np.ndarray
-    right: np.ndarray
+    kind: str = "sod"                 # "sod" (shock tube) or "sedov" (central energy)
+    normal: np.ndarray | None = None  # sod: plane normal
+    left: np.ndarray | None = None    # sod: primitive state on the normal's negative side
+    right: np.ndarray | None = None
+    energy: float = 0.0               # sedov: deposited energy
 
 
-def shock_tube_ensemble(nbox: int = 96, seed: int = 20261202, side_min: int = 16, side_max: int = 64,
-                        gamma: float = 1.4, boxlen: float = 1.0) -> list[EnsembleBox]:
-    """Independent periodic cubic boxes, each a shock tube through its centre.
-    Sides are drawn log-uniformly in [side_min, side_max] from
+def box_ensemble(nbox: int = 96, seed: int = 20261202, side_min: int = 16, side_max: int = 64, gamma: float = 1.4,
+                 boxlen: float = 1.0, sedov_share: float = 0.5) -> list[EnsembleBox]:
+    """Independent periodic cubic boxes of the batched benchmark.  Sides are
+    drawn log-uniformly in [side_min, side_max] from
     numpy.random.default_rng(seed) and rounded (in log) to the nearest
-    nx * 2^(L-1); the plane normal is uniform on the sphere; each side's
-    density and pressure are uniform in [0.1, 1], at rest.  The solver's cell
-    width is dx = boxlen / (nx * 2^L), as the reference passes it
-    (Simulation.cpp:548)."""
+    nx * 2^(L-1).  With probability sedov_share a box is a Sedov blast: rho = 1,
+    p = 1e-5 at rest, and an energy drawn uniformly in [0.1, 1] deposited as
+    pressure in the 8 cells within 1.5 cell widths of the centre
+    (Initializer.cpp:87-94, the set-up of sedov_regions); otherwise it is a
+    shock tube through the centre, the plane normal uniform on the sphere and
+    each side's density and pressure uniform in [0.1, 1], at rest.  The
+    solver's cell width is dx = boxlen / (nx * 2^L), as the reference passes
+    it (Simulation.cpp:548).  With sedov_share = 0 no kind is drawn, so the
+    boxes are those of shock_tube_ensemble for the same seed."""
     rng = np.random.default_rng(seed)
     table = _quantised_sides(side_min, side_max)
     logs = np.log([t[0] for t in table])
@@ -403,15 +410,27 @@ def shock_tube_ensemble(nbox: int = 96, seed: int = 20261202, side_min: int = 16
     for _ in range(nbox):
         s = rng.uniform(np.log(side_min), np.log(side_max))
         side, nx, L = table[int(np.argmin(np.abs(logs - s)))]
+        m = uniform_mesh(L, nx=nx, ny=nx, nz=nx)
+        dx = boxlen / (nx * 2 ** L)
+        if sedov_share > 0 and rng.uniform() < sedov_share:
+            e = float(rng.uniform(0.1, 1.0))
+            init_regions(m, sedov_regions(L, nx=nx, gamma=gamma, energy=e, boxlen=boxlen), gamma, boxlen)
+            boxes.append(EnsembleBox(m, nx, L, dx, kind="sedov", energy=e))
+            continue
         n = rng.normal(size=3)
         n /= np.linalg.norm(n)
         dl, pl, dr, pr = rng.uniform(0.1, 1.0, 4)
         left = np.array([dl, 0.0, 0.0, 0.0, pl])
         right = np.array([dr, 0.0, 0.0, 0.0, pr])
-        m = uniform_mesh(L, nx=nx, ny=nx, nz=nx)
         init_shock_tube(m, n, np.full(3, 0.5 * boxlen), left, right, gamma, boxlen)
-        boxes.append(EnsembleBox(m, nx, L, boxlen / (nx * 2 ** L), n, left, right))
+        boxes.append(EnsembleBox(m, nx, L, dx, kind="sod", normal=n, left=left, right=right))
     return boxes
+
+
+def shock_tube_ensemble(nbox: int = 96, seed: int = 20261202, side_min: int = 16, side_max: int = 64,
+                        gamma: float = 1.4, boxlen: float = 1.0) -> list[EnsembleBox]:
+    """The shock-tube boxes of box_ensemble only (sedov_share = 0)."""
+    return box_ensemble(nbox, seed, side_min, side_max, gamma, boxlen, sedov_share=0.0)
 
 
 class RhError(RuntimeError):

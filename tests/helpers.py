@@ -7,6 +7,10 @@ import numpy as np
 import oracle
 import rhydro
 
+# Every Godunov kernel variant (per-cell, oct-in-lanes, bricks); all must give
+# identical results.
+VARIANTS = list(range(rhydro.RH_KERNEL_VARIANTS))
+
 
 def clone(mesh: rhydro.Mesh) -> rhydro.Mesh:
     m = copy.copy(mesh)

@@ -8,7 +8,7 @@ import pytest
 import oracle
 import rhydro
 from amr_meshes import refined_mesh
-from helpers import assert_same, clone
+from helpers import VARIANTS, assert_same, clone
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +39,7 @@ class _OracleApi:
         self.om.godunov_fine(self.p, L, dt, dx)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("riemann", ["llf", "hllc"])
 @pytest.mark.parametrize("mode", [rhydro.RH_DEVICE_RESIDENT, rhydro.RH_HOST_COHERENT])
 def test_two_level_refinement(riemann, mode, variant):
@@ -59,7 +59,7 @@ def test_two_level_refinement(riemann, mode, variant):
     s.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("slope_type,interpol_type", [(2, 1), (1, 2), (8, 3), (0, 0)])
 def test_single_call_on_each_amr_level(slope_type, interpol_type, variant):
     m0 = refined_mesh(4, SPHERES, seed=11, interpol_type=interpol_type)
@@ -114,7 +114,7 @@ def test_rebind_after_refinement():
     s.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", VARIANTS)
 def test_checkerboard_refinement_many_reflux_targets(variant):
     """Every other cell of a 32^3 level refined in a 3D checkerboard: each
     fine oct is surrounded by coarser leaf cells on all six faces, and the

@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 import rhydro
-from helpers import clone, oracle_params
+from helpers import VARIANTS, clone, oracle_params
 
 pytestmark = pytest.mark.gpu
 
@@ -100,7 +100,7 @@ def test_mixed_variants_identical(riemann):
     m0 = rhydro.uniform_mesh(level)
     rhydro.init_lognormal(m0, level, 321, 1.4, 0.5, 0.5)
     outs = []
-    for variant in (0, 1, 2):
+    for variant in VARIANTS:
         print(f"case mixed variant {variant} {riemann}", flush=True)
         m = clone(m0)
         s = rhydro.Solver(gamma=1.4, slope_type=2, riemann=riemann, precision=rhydro.RH_PREC_MIXED, variant=variant)

@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 import rhydro
-from helpers import assert_same, clone, oracle_params, step_gpu, step_oracle
+from helpers import VARIANTS, assert_same, clone, oracle_params, step_gpu, step_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +39,7 @@ def _compare_steps(m0, solver_kw, level, nsteps, cf=0.8, mode=rhydro.RH_DEVICE_R
     s.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("riemann", RIEMANNS)
 @pytest.mark.parametrize("slope_type", range(9))
 def test_step_riemann_slope(riemann, slope_type, variant):
@@ -53,7 +53,7 @@ def test_step_interpol_type(interpol_type):
     _compare_steps(m, dict(gamma=1.4, slope_type=2, riemann="hllc", interpol_type=interpol_type), 4, 2)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("level", [2, 3, 5])
 def test_small_and_degenerate_levels(level, variant):
     # level 2 is a single oct whose every neighbour is itself (periodic).
@@ -80,7 +80,7 @@ def test_sedov_32_trajectory():
     _compare_steps(m, dict(gamma=1.4, slope_type=1, riemann="hllc"), level, 10)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", VARIANTS)
 def test_sedov_64_run_uniform_20_steps(variant):
     """Config 1 at its stated size (64^3, reference level 7), 20 steps
     through rh_run_uniform, against the oracle step by step."""
@@ -270,7 +270,7 @@ def test_run_uniform_exchanged_buffers_match_copies(precision):
     m0 = _random_mesh(level, seed=77)
     for n in (1, 2, 3):
         out = []
-        for variant in (0, 1, 2):
+        for variant in VARIANTS:
             m = clone(m0)
             s = rhydro.Solver(gamma=5.0 / 3.0, slope_type=1, riemann="hllc", precision=precision, variant=variant)
             s.bind(m)
@@ -279,7 +279,7 @@ def test_run_uniform_exchanged_buffers_match_copies(precision):
             s.sync_to_host()
             s.close()
             out.append((np.array(dts), m.uold.copy(), m.unew.copy()))
-        for v in (1, 2):
+        for v in VARIANTS[1:]:
             assert_same(out[v][0], out[0][0], f"dt, {n} steps, variant {v}")
             assert_same(out[v][1], out[0][1], f"uold, {n} steps, variant {v}")
             assert_same(out[v][2], out[0][2], f"unew, {n} steps, variant {v}")
