@@ -366,6 +366,66 @@ def measure_sedov64(args, rhydro, np):
     return out
 
 
+def measure_dropin(args, rhydro, np, which):
+    """The drop-in path as the HydroSolver adapter drives it
+    (ramses-cpp_amd/adapter/PatchHydroSolver.cpp), one amr_step of a uniform
+    level = compute_courant_step, set_unew, godunov_fine, set_uold
+    (Simulation.cpp:550,564,589,614), each through its own C-ABI call:
+      host-coherent (what an adaptive run needs): every call copies uold and
+        unew to the GPU and the array it wrote back, and the adapter first
+        compares the topology arrays with its copy from the last binding;
+      device-resident (the adapter's mode for uniform runs): the state stays
+        on the GPU and uold comes back once per step, after set_uold.
+    Both leave the same state (compared at the end).  `which`: "c3" is the
+    256^3 lognormal workload of the main measurement, "c1" the 64^3 Sedov
+    blast."""
+    if which == "c3":
+        level, gamma, slope, nsteps = 9, 5.0 / 3.0, 1, 3
+        mesh = rhydro.uniform_mesh(level)
+        rhydro.init_lognormal(mesh, level, args.seed, gamma, 0.5, 0.5)
+        name = "256^3 lognormal workload (reference level 9), gamma=5/3, HLLC, slope_type 1"
+    else:
+        level, gamma, slope, nsteps = 7, 1.4, 1, 10
+        mesh = rhydro.uniform_mesh(level)
+        rhydro.init_regions(mesh, rhydro.sedov_regions(level), gamma)
+        name = "Sedov blast 64^3 (reference level 7), gamma=1.4, HLLC, slope_type 1"
+    dx = 1.0 / 2 ** level
+    u0 = mesh.uold.copy()
+    out = {"workload": name + ", one C-ABI call per HydroSolver virtual, FP64 form", "steps": nsteps}
+    finals = {}
+    for mode_name, mode in (("host_coherent", rhydro.RH_HOST_COHERENT), ("device_resident", rhydro.RH_DEVICE_RESIDENT)):
+        mesh.uold[:] = u0
+        mesh.unew[:] = u0
+        s = rhydro.Solver(gamma=gamma, slope_type=slope, riemann="hllc")
+        s.bind(mesh, mode)
+        nleaf = s.leaf_cells(level)
+        topo = [mesh.arrays[k] for k in ("father", "son", "nbor", "next", "headl")]
+        saved = [a.copy() for a in topo]
+        calls = 0.0
+        t0 = time.perf_counter()
+        for k in range(nsteps):
+            for _ in range(4 if mode == rhydro.RH_HOST_COHERENT else 0):
+                # the adapter's per-call topology test (memcmp of the saved arrays)
+                assert all(np.array_equal(a, b) for a, b in zip(topo, saved))
+            tc = time.perf_counter()
+            dt = s.courant(level, dx, gamma, 0.8)
+            s.set_unew(level)
+            s.godunov_fine(level, dt, dx)
+            s.set_uold(level)
+            if mode == rhydro.RH_DEVICE_RESIDENT:
+                s.sync_to_host(rhydro.RH_UOLD)
+            calls += time.perf_counter() - tc
+        wall = time.perf_counter() - t0
+        s.close()
+        finals[mode_name] = mesh.uold.copy()
+        out[mode_name] = {"value": nleaf * nsteps / wall, "ms_per_step": wall / nsteps * 1e3,
+                          "topology_check_ms_per_step": (wall - calls) / nsteps * 1e3}
+        print(f"[bench] drop-in {which} {mode_name}: {wall / nsteps * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
+    out["cells"] = int(nleaf)
+    out["modes_identical"] = bool(np.array_equal(finals["host_coherent"], finals["device_resident"]))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -381,6 +441,7 @@ def main() -> int:
     ap.add_argument("--boxes", type=int, default=96, help="boxes of the shock-tube and Sedov ensemble (0: skip it)")
     ap.add_argument("--no-variants", action="store_true", help="skip the HLL / slope-2 variants of the workload")
     ap.add_argument("--no-sedov", action="store_true", help="skip the 64^3 Sedov run")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in (per-virtual C-ABI call) measurement")
     ap.add_argument("--only", choices=["ensemble", "amr_sedov", "amr_kh", "sedov64"],
                     help="run one secondary workload alone and print its record (for profiling)")
     ap.add_argument("--amr-steps", type=int, default=50, help="coarse steps of the AMR Sedov run (0: skip)")
@@ -508,6 +569,7 @@ def main() -> int:
             variants.append(row)
 
     ens_info = measure_ensemble(args, rhydro, np) if args.boxes > 0 else None
+    dropin = None if args.no_dropin else {w: measure_dropin(args, rhydro, np, w) for w in ("c3", "c1")}
     sedov_info = measure_sedov64(args, rhydro, np) if not args.no_sedov else None
     amr_sedov = measure_amr(args, rhydro, np, "sedov") if args.amr_steps > 0 else None
     amr_kh = measure_amr(args, rhydro, np, "kh") if args.kh_steps > 0 else None
@@ -585,6 +647,7 @@ def main() -> int:
             "mixed": mixed_info,
             "variants": variants,
             "ensemble": ens_info,
+            "drop_in": dropin,
             "sedov64": sedov_info,
             "amr_sedov": amr_sedov,
             "amr_kelvin_helmholtz": amr_kh,

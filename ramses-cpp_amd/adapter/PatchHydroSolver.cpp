@@ -8,10 +8,18 @@
 // include/rhydro.h.
 //
 // The reference mutates the AmrGrid on the host between calls (refinement,
-// restriction, flagging), so the context runs in RH_HOST_COHERENT mode: state
-// is copied in before and out after each call, and the mesh is re-bound
-// whenever the topology arrays change (detected by pointer and checksum).
+// restriction, flagging), so an adaptive run's context runs in
+// RH_HOST_COHERENT mode: state is copied in before and out after each call,
+// and the mesh is re-bound whenever the topology arrays change (detected by
+// comparing them with the copy taken at the last binding, and by pointer).
+// A uniform run (levelmin == levelmax) with no host-side physics between the
+// hydro calls keeps the state on the GPU instead: nothing on the host reads
+// or writes uold/unew between those calls there (amr_step neither refines,
+// restricts nor flags), so the context is device-resident, the topology is
+// bound once, and only uold comes back, once per step after set_uold, for
+// the snapshots and diagnostics that read it.
 #include <cstdint>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -40,6 +48,7 @@ public:
     void set_uold(int ilevel) override {
         ensure_bound();
         check(rh_set_uold(ctx_, ilevel), "rh_set_uold");
+        if (resident_) check(rh_sync(ctx_, RH_SYNC_TO_HOST, RH_UOLD), "rh_sync");
     }
 
     void set_unew(int ilevel) override {
@@ -72,8 +81,10 @@ public:
 private:
     rh_ctx* ctx_ = nullptr;
     long long epoch_ = 0;
-    uint64_t topo_sum_ = 0;
-    const void* topo_ptrs_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool resident_ = false;  // uniform run: state kept on the GPU
+    // Topology arrays and state pointers at the last binding.
+    std::vector<int> topo_copy_[5];
+    const void* ptrs_[8] = {};
 
     void check(int st, const char* what) {
         if (st != RH_OK) throw std::runtime_error(std::string(what) + ": " + rh_last_error(ctx_));
@@ -103,31 +114,44 @@ private:
         p.precision = RH_PREC_FP64;
         p.device = 0;
         check(rh_create(&p, &ctx_), "rh_create");
+        // Host code between the hydro calls of amr_step (Simulation.cpp:
+        // 522-635): refinement, restriction and flagging only when levelmin <
+        // levelmax; cooling, turbulence forcing, sinks and stars when enabled.
+        const bool uniform = config_.get_int("amr_params", "levelmin", 1) == config_.get_int("amr_params", "levelmax", 1);
+        const bool host_physics = config_.get_bool("cooling_params", "cooling", false) ||
+                                  config_.get_bool("run_params", "turb", false) ||
+                                  config_.get_bool("run_params", "sink", false) ||
+                                  config_.get_bool("run_params", "star", false);
+        resident_ = uniform && !host_physics;
     }
 
-    static uint64_t fnv(uint64_t h, const std::vector<int>& v) {
-        for (int x : v) {
-            h ^= (uint32_t)x;
-            h *= 1099511628211ull;
-        }
-        return h;
+    // True when the topology or any array pointer differs from the last
+    // binding (then the copy is refreshed).  memcmp of the saved arrays: an
+    // exact test that streams them at memory bandwidth.
+    bool topology_changed() {
+        const std::vector<int>* arrs[5] = {&grid_.father, &grid_.son, &grid_.nbor, &grid_.next, &grid_.headl_vec};
+        const void* ptrs[8] = {grid_.father.data(), grid_.son.data(),      grid_.nbor.data(),     grid_.next.data(),
+                               grid_.headl_vec.data(), grid_.uold_vec.data(), grid_.unew_vec.data(), grid_.f_vec.data()};
+        bool changed = epoch_ == 0;
+        for (int i = 0; i < 8 && !changed; ++i) changed = ptrs[i] != ptrs_[i];
+        for (int i = 0; i < 5 && !changed; ++i)
+            changed = arrs[i]->size() != topo_copy_[i].size() ||
+                      std::memcmp(arrs[i]->data(), topo_copy_[i].data(), arrs[i]->size() * sizeof(int)) != 0;
+        if (!changed) return false;
+        for (int i = 0; i < 5; ++i) topo_copy_[i] = *arrs[i];
+        for (int i = 0; i < 8; ++i) ptrs_[i] = ptrs[i];
+        return true;
     }
 
     void ensure_bound() {
         ensure_context();
-        const void* ptrs[5] = {grid_.father.data(), grid_.son.data(), grid_.nbor.data(), grid_.next.data(),
-                               grid_.headl_vec.data()};
-        uint64_t sum = 1469598103934665603ull;
-        sum = fnv(sum, grid_.father);
-        sum = fnv(sum, grid_.son);
-        sum = fnv(sum, grid_.nbor);
-        sum = fnv(sum, grid_.next);
-        sum = fnv(sum, grid_.headl_vec);
-        bool same = sum == topo_sum_ && epoch_ > 0;
-        for (int i = 0; i < 5; ++i) same = same && ptrs[i] == topo_ptrs_[i];
-        if (same) return;
-        for (int i = 0; i < 5; ++i) topo_ptrs_[i] = ptrs[i];
-        topo_sum_ = sum;
+        // A device-resident (uniform) run never changes its topology: it is
+        // bound once.
+        if (resident_) {
+            if (epoch_ > 0) return;
+        } else if (!topology_changed()) {
+            return;
+        }
         ++epoch_;
         rh_mesh m{};
         m.ncoarse = grid_.ncoarse;
@@ -150,7 +174,7 @@ private:
         m.unew = grid_.unew_vec.data();
         m.f = grid_.f_vec.size() == (size_t)3 * grid_.ncell ? grid_.f_vec.data() : nullptr;
         m.mesh_epoch = epoch_;
-        check(rh_bind(ctx_, &m, RH_HOST_COHERENT), "rh_bind");
+        check(rh_bind(ctx_, &m, resident_ ? RH_DEVICE_RESIDENT : RH_HOST_COHERENT), "rh_bind");
     }
 };
 

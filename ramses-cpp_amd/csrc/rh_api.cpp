@@ -39,9 +39,17 @@ int hip_fail(rh_ctx* c, hipError_t e, const char* what) {
 }
 
 
+// Callers have synchronised the stream (no queued kernel reads the tables).
 void free_level(DevLevel& L) {
     for (void* p : L.allocs) (void)hipFree(p);
     L.allocs.clear();
+    if (L.staged) (void)hipEventSynchronize(L.staged);
+    if (L.dblock) (void)hipFree(L.dblock);
+    if (L.hblock) (void)hipHostFree(L.hblock);
+    if (L.staged) (void)hipEventDestroy(L.staged);
+    L.dblock = L.hblock = nullptr;
+    L.dcap = L.hcap = 0;
+    L.staged = nullptr;
     L.dev = rh::LevelDev{};
     L.epoch = -2;
     L.ngridmax = L.ncell = -1;
@@ -94,35 +102,84 @@ int get_level(rh_ctx* c, int ilevel, const rh::LevelDev** out) {
             if (st != RH_OK) return fail(c, st, e);
             c->topo_valid = true;
         }
-        free_level(L);
         rh::LevelHost h;
         std::string e;
         int st = rh::build_level(c->mesh, c->topo, ilevel, h, e);
         if (st == RH_OK) st = rh::validate_level(h, c->mesh.ngridmax, c->mesh.ncell, e);
-        if (st != RH_OK) return fail(c, st, e);
+        if (st != RH_OK) {
+            L.epoch = -2;
+            return fail(c, st, e);
+        }
+        const int nrec = (int)h.rec_n.size();
+        std::vector<int4> rec(nrec);
+        for (int r = 0; r < nrec; ++r) rec[r] = make_int4(h.rec_n[r], h.rec_nl[r], h.rec_nr[r], h.rec_flags[r]);
+        // Layout of the block: each table at a 256-byte boundary, the reflux
+        // slots (device only) last.
+        struct Part {
+            const void* src;
+            size_t bytes;
+            size_t off;
+        };
+        Part parts[8] = {{h.grid.data(), h.grid.size() * sizeof(int), 0},
+                         {h.leaf.data(), h.leaf.size(), 0},
+                         {h.nbo.data(), h.nbo.size() * sizeof(int), 0},
+                         {rec.data(), rec.size() * sizeof(int4), 0},
+                         {h.rec_slot.data(), h.rec_slot.size() * sizeof(int), 0},
+                         {h.tgt_cell.data(), h.tgt_cell.size() * sizeof(int), 0},
+                         {h.tgt_off.data(), h.tgt_off.size() * sizeof(int), 0},
+                         {h.tgt_slot.data(), h.tgt_slot.size() * sizeof(int), 0}};
+        size_t up = 0;
+        for (Part& pt : parts) {
+            pt.off = up;
+            up += (pt.bytes + 255) & ~(size_t)255;
+        }
+        const size_t slot_bytes = (size_t)h.nslots * 5 * sizeof(double);
+        const size_t total = up + slot_bytes;
+        if (!L.staged) RH_HIP(c, hipEventCreateWithFlags(&L.staged, hipEventDisableTiming));
+        // The previous copy out of the host block has finished before it is
+        // refilled (normally long before).
+        RH_HIP(c, hipEventSynchronize(L.staged));
+        if (up > L.hcap) {
+            if (L.hblock) (void)hipHostFree(L.hblock);
+            L.hblock = nullptr;
+            L.hcap = 0;
+            const size_t cap = up + up / 4 + 4096;
+            RH_HIP(c, hipHostMalloc(&L.hblock, cap, hipHostMallocDefault));
+            L.hcap = cap;
+        }
+        if (total > L.dcap) {
+            // Queued kernels may still read the old block.
+            RH_HIP(c, hipStreamSynchronize(c->stream));
+            if (L.dblock) (void)hipFree(L.dblock);
+            L.dblock = nullptr;
+            L.dcap = 0;
+            const size_t cap = total + total / 4 + 4096;
+            RH_HIP(c, hipMalloc(&L.dblock, cap));
+            L.dcap = cap;
+        }
+        for (const Part& pt : parts)
+            if (pt.bytes) std::memcpy((char*)L.hblock + pt.off, pt.src, pt.bytes);
+        // Stream-ordered after every kernel queued on the old tables.
+        RH_HIP(c, hipMemcpyAsync(L.dblock, L.hblock, up, hipMemcpyHostToDevice, c->stream));
+        RH_HIP(c, hipEventRecord(L.staged, c->stream));
+        char* base = (char*)L.dblock;
         rh::LevelDev& d = L.dev;
+        d = rh::LevelDev{};
         d.noct = h.noct;
-        d.nrec = (int)h.rec_n.size();
+        d.nrec = nrec;
         d.nslots = h.nslots;
         d.ntgt = (int)h.tgt_cell.size();
         d.nleaf = h.nleaf;
-        std::vector<int4> rec(d.nrec);
-        for (int r = 0; r < d.nrec; ++r) rec[r] = make_int4(h.rec_n[r], h.rec_nl[r], h.rec_nr[r], h.rec_flags[r]);
-        RH_HIP(c, upload(L, d.grid, h.grid, c->stream));
-        RH_HIP(c, upload(L, d.leaf, h.leaf, c->stream));
-        RH_HIP(c, upload(L, d.nbo, h.nbo, c->stream));
-        RH_HIP(c, upload(L, d.rec, rec, c->stream));
-        RH_HIP(c, upload(L, d.rec_slot, h.rec_slot, c->stream));
-        RH_HIP(c, upload(L, d.tgt_cell, h.tgt_cell, c->stream));
-        RH_HIP(c, upload(L, d.tgt_off, h.tgt_off, c->stream));
-        RH_HIP(c, upload(L, d.tgt_slot, h.tgt_slot, c->stream));
-        if (d.nslots > 0) {
-            void* p = nullptr;
-            RH_HIP(c, hipMalloc(&p, (size_t)d.nslots * 5 * sizeof(double)));
-            L.allocs.push_back(p);
-            d.slot_r = (double*)p;
-        }
-        RH_HIP(c, hipStreamSynchronize(c->stream));  // host vectors go out of scope
+        auto at = [&](int k) { return parts[k].bytes ? (void*)(base + parts[k].off) : nullptr; };
+        d.grid = (int*)at(0);
+        d.leaf = (uint8_t*)at(1);
+        d.nbo = (int*)at(2);
+        d.rec = (int4*)at(3);
+        d.rec_slot = (int*)at(4);
+        d.tgt_cell = (int*)at(5);
+        d.tgt_off = (int*)at(6);
+        d.tgt_slot = (int*)at(7);
+        d.slot_r = slot_bytes ? (double*)(base + up) : nullptr;
         L.epoch = c->mesh.mesh_epoch;
         L.ngridmax = c->mesh.ngridmax;
         L.ncell = c->mesh.ncell;

@@ -21,7 +21,17 @@ struct DevLevel {
     int ngridmax = -1;
     int ncell = -1;
     rh::LevelDev dev;
-    std::vector<void*> allocs;
+    std::vector<void*> allocs;  // separately allocated tables (ensemble form)
+    // A single mesh's tables: one device block (the tables, then the reflux
+    // slots) filled from one pinned host block by one copy, both kept and
+    // reused across rebuilds while large enough, so a rebuild neither waits
+    // for the stream nor frees device memory.  `staged` is recorded after the
+    // copy from hblock; the next rebuild waits on it before refilling hblock.
+    void* dblock = nullptr;
+    size_t dcap = 0;
+    void* hblock = nullptr;
+    size_t hcap = 0;
+    hipEvent_t staged = nullptr;
 };
 
 enum { TK_GODUNOV = 0, TK_COURANT = 1, TK_SET_UNEW = 2, TK_SET_UOLD = 3, TK_STEP_JOIN = 4, TK_N = 5 };

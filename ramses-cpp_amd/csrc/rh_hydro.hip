@@ -17,9 +17,12 @@
 //                             cross-lane shuffle, ordered accumulation in LDS)
 //   k_coarse_reflux(_wave)    ordered reflux into coarser leaf cells
 //   k_interpol / k_new_octs   prolongation of coarse cells into new octs
-//   k_restrict, k_set_uold_restrict, k_grad_flags, k_amr_times
-//                             adaptive-run restriction, gradient flags and
-//                             subcycling time-step bookkeeping
+//   k_restrict, k_set_uold_restrict, k_amr_times
+//                             adaptive-run restriction and subcycling
+//                             time-step bookkeeping
+//   k_smooth_bits, k_grad_bits
+//                             flag_fine's smoothing passes and gradient
+//                             criteria over a level list
 //   k_gravity_source / k_synchro, k_pack_hydro
 //                             gravity kicks; snapshot packing
 //
@@ -35,6 +38,7 @@
 #include <cstdint>
 #include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "rh_kernels.h"
 #include "rh_physics.h"
@@ -781,7 +785,8 @@ __global__ __launch_bounds__(G_BLOCK) void k_godunov(MeshDev m, LevelDev lv, Gas
 // registers (negative: none).  Slot f's contribution is the reference's
 // ((dtdx * sign) * F) * 0.125 with the neighbour's face sign: -1 for a
 // neighbour on the minus side (its plus face), +1 on the plus side.
-template <int NT, typename TF>
+// With CONTRIB the slots already hold the FP64 contributions themselves.
+template <int NT, bool CONTRIB = false, typename TF>
 RH_DEV void ordered_apply_flux(const MeshDev& m, int cC, long long own_key, const double fs[5], double dtdx,
                                const GasParams& gp, const TF (*s_f)[NT], const long long kin[6], int t,
                                const double* unew_pre, double* u_out) {
@@ -799,7 +804,10 @@ RH_DEV void ordered_apply_flux(const MeshDev& m, int cC, long long own_key, cons
     for (int k = 0; k < 6; ++k) {
         const double sdt = dtdx * ((ix[k] & 1) ? 1.0 : -1.0);
 #pragma unroll
-        for (int iv = 0; iv < 5; ++iv) rs[k][iv] = sdt * (double)s_f[ix[k] * 5 + iv][t] * 0.125;
+        for (int iv = 0; iv < 5; ++iv) {
+            const double v = (double)s_f[ix[k] * 5 + iv][t];
+            rs[k][iv] = CONTRIB ? v : sdt * v * 0.125;
+        }
     }
     double u[5];
     copy5(unew_pre, u);
@@ -881,8 +889,9 @@ RH_DEV int xcd_block(int x, int n) {
 template <int OCTS> constexpr int oct_block() { return OCTS * 8; }
 
 // Occupancy: the FP64 form runs two waves per SIMD (at most 256 VGPRs), the
-// mixed form three (at most 168).  LDS per oct: the incoming face fluxes in
-// the compute type, 1.9 KB in the FP64 form and 0.9 KB in the mixed form.
+// mixed form three (at most 168).  LDS per oct: the incoming contributions
+// and their keys, 2.25 KB in the FP64 form, and the FP32 incoming fluxes,
+// 0.9 KB, in the mixed form.
 // The registers above what the arithmetic needs hold the next dimension's
 // neighbour states, gathered ahead of their use.
 //
@@ -902,8 +911,14 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                                                           double courant_factor) {
     using TC = typename std::conditional<MIXED, float, double>::type;
     constexpr int NT = oct_block<OCTS>();
-    // The six incoming face fluxes of each cell (slot f*5 + iv).
+    // The six incoming face contributions of each cell (slot f*5 + iv).
+    // The FP64 form stores each contribution ((dtdx*sign)*F)*0.125 and its
+    // key as it is found; the mixed form keeps the FP32 flux F (the
+    // contribution is formed in FP64 when it is applied) and recomputes the
+    // keys from the tables at the end, which halves its LDS so that three
+    // waves share a SIMD.
     __shared__ TC s_f[30][NT];
+    __shared__ long long s_k[MIXED ? 1 : 6][NT];
     const int t = threadIdx.x;
     const int ic0 = t & 7;
     const int praw = xcd_block(blockIdx.x, gridDim.x) * OCTS + (t >> 3);
@@ -1037,6 +1052,7 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         // External face (across the oct boundary).
         TC Fext[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
         TC Fin[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+        long long Kin = -1;
         if (E >= 0) {
             if (cur.X2cell > 0) {
                 TC qX2[5];
@@ -1056,6 +1072,7 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
                 sel5(par, OXp, GCp, qr);
                 solve_face<RIEMANN, MIXED>(ql, qr, Fin, gc, gd);
                 rotate(Fin, d);
+                Kin = ((long long)E * 8 + icS) * 8 + (par ? 2 * d : 2 * d + 1);
             }
         } else if (rc.w & REC_REFINED_DEV) {
         } else if (rc.w & REC_BOUNDARY_DEV) {
@@ -1108,23 +1125,40 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
         // face when it is on the minus side) and X's.
         const int slotS = 2 * d + (par ? 0 : 1);
         const int slotX = 2 * d + (par ? 1 : 0);
+        if constexpr (MIXED) {
 #pragma unroll
-        for (int iv = 0; iv < 5; ++iv) {
-            s_f[5 * slotS + iv][t] = Fsib[iv];
-            s_f[5 * slotX + iv][t] = Fin[iv];
+            for (int iv = 0; iv < 5; ++iv) {
+                s_f[5 * slotS + iv][t] = Fsib[iv];
+                s_f[5 * slotX + iv][t] = Fin[iv];
+            }
+        } else {
+            const double sgS = par ? -1.0 : 1.0;
+            const double sgX = par ? 1.0 : -1.0;
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) {
+                s_f[5 * slotS + iv][t] = dtdx * sgS * Fsib[iv] * 0.125;
+                s_f[5 * slotX + iv][t] = dtdx * sgX * Fin[iv] * 0.125;
+            }
+            s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
+            s_k[slotX][t] = Kin;
         }
     }
     long long kin[6];
-    incoming_keys(lv, p, ic0, lm, kin);
+    if constexpr (MIXED) {
+        incoming_keys(lv, p, ic0, lm, kin);
+    } else {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) kin[f] = s_k[f][t];
+    }
     if (!cpart && !ckeys) {
         if (!leaf) return;
-        ordered_apply_flux<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, uXn, nullptr);
+        ordered_apply_flux<NT, !MIXED>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, uXn, nullptr);
         return;
     }
     double cand = __builtin_inf();
     if (leaf) {
         double un[5];
-        ordered_apply_flux<NT>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, uXn, un);
+        ordered_apply_flux<NT, !MIXED>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, uXn, un);
         cand = courant_cand_u(m, un, cC, dx, gp.gamma, courant_factor);
     }
     if (cpart) {
@@ -1195,14 +1229,15 @@ RH_DEV int nth_set_bit(unsigned long long mask, int k) {
 
 template <int OCTS> constexpr int brick_block() { return OCTS * 8; }
 
-// Occupancy: two waves per SIMD (at most 256 VGPRs).  LDS per lane: six
-// incoming face fluxes and one ghost state in the compute type (280 B in the
-// FP64 form, 140 B in the mixed form), plus 16 B of halo cell ids: 148 KB for
-// a 64-oct workgroup in the FP64 form, so one such workgroup fills a CU; 32-
-// and 16-oct workgroups let two or four share one.
+// Occupancy: two waves per SIMD in the FP64 form (at most 256 VGPRs), three
+// in the mixed form (at most 168).  LDS per lane: six incoming face fluxes
+// and one ghost state in the compute type (280 B in the FP64 form, 140 B in
+// the mixed form), plus 16 B of halo cell ids: 148 KB for a 64-oct workgroup
+// in the FP64 form, so one such workgroup fills a CU; 32- and 16-oct
+// workgroups let two or four share one.
 // `pre`, cpart, ckeys and courant_factor as in k_godunov_oct.
 template <int RIEMANN, bool MIXED, bool EXACT, int OCTS>
-__global__ __launch_bounds__(OCTS * 8) __attribute__((amdgpu_waves_per_eu(2, 8)))
+__global__ __launch_bounds__(OCTS * 8) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2, 8)))
 void k_godunov_brick(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt, double dx, double inv_dx,
                      const double* pre, double* cpart, unsigned long long* ckeys, double courant_factor) {
     using TC = typename std::conditional<MIXED, float, double>::type;
@@ -1891,42 +1926,86 @@ RH_DEV double err_grad_u(double vl, double vc, double vr, double pl, double pc, 
                       sabs(vc - vl) / (cm + cg + sabs(vc) + sabs(vl) + floor));
 }
 
+// Refinement flags of a level list on the device, one byte per oct in list
+// order (bit ic-1: cell ic is flagged), so that flag_fine's passes over the
+// list (TreeUpdater.cpp:328-582) run where the state and the level tables
+// are and only the final flags go back to the host tree.
+
 // Gradient criteria of flag_fine (TreeUpdater.cpp:482-526) for every cell of
-// a level list: out[8*pos + ic-1] = 1 when a density, pressure or velocity
-// error exceeds its threshold along some direction.  The neighbour along a
-// face is the same-level cell when its oct exists (get_nbor_cells_exact),
-// else the grid's nbor entry; a neighbour outside 1..ncell is replaced by
-// the cell itself.
-__global__ __launch_bounds__(256) void k_grad_flags(MeshDev m, LevelDev lv, const int* __restrict__ nbor,
-                                                    double ed, double ep, double ev, double gamma,
-                                                    unsigned char* __restrict__ out) {
+// a level list, OR-ed into the oct's flag byte: a cell is flagged when a
+// density, pressure or velocity error exceeds its threshold along some
+// direction (the reference tests only unflagged cells; testing every cell and
+// OR-ing gives the same flags).  The neighbour along a face is the same-level
+// cell when its oct exists (get_nbor_cells_exact), else the grid's nbor
+// entry; a neighbour outside 1..ncell is replaced by the cell itself.  The
+// eight cells of an oct are eight adjacent lanes; the first writes the byte.
+__global__ __launch_bounds__(256) void k_grad_bits(MeshDev m, LevelDev lv, const int* __restrict__ nbor, double ed,
+                                                   double ep, double ev, double gamma, unsigned char* __restrict__ bits) {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= lv.noct * 8) return;
-    const int p = k >> 3, ic0 = k & 7;
-    const int g = lv.grid[p];
-    const int c = cell_id(m, g, ic0);
-    const FlagState sc = flag_state(m, c, gamma);
+    const bool valid = k < lv.noct * 8;
+    const int p = valid ? k >> 3 : lv.noct - 1, ic0 = k & 7;
     bool ok = false;
-    for (int d = 0; d < 3 && !ok; ++d) {
-        int idn[2];
-        for (int side = 0; side < 2; ++side) {
-            const int par = (ic0 >> d) & 1;
-            if (par != side) {
-                idn[side] = cell_id(m, g, ic0 ^ (1 << d));  // sibling inside the oct
-            } else {
-                const int E = lv.nbo[(size_t)(2 * d + side) * lv.noct + p];
-                idn[side] = E >= 0 ? cell_id(m, lv.grid[E], ic0 ^ (1 << d))
-                                   : nbor[(size_t)(2 * d + side) * m.ngridmax + g - 1];
+    if (valid) {
+        const int g = lv.grid[p];
+        const int c = cell_id(m, g, ic0);
+        const FlagState sc = flag_state(m, c, gamma);
+        for (int d = 0; d < 3 && !ok; ++d) {
+            int idn[2];
+            for (int side = 0; side < 2; ++side) {
+                const int par = (ic0 >> d) & 1;
+                if (par != side) {
+                    idn[side] = cell_id(m, g, ic0 ^ (1 << d));  // sibling inside the oct
+                } else {
+                    const int E = lv.nbo[(size_t)(2 * d + side) * lv.noct + p];
+                    idn[side] = E >= 0 ? cell_id(m, lv.grid[E], ic0 ^ (1 << d))
+                                       : nbor[(size_t)(2 * d + side) * m.ngridmax + g - 1];
+                }
             }
+            const FlagState sl = (idn[0] > 0 && idn[0] <= m.ncell) ? flag_state(m, idn[0], gamma) : sc;
+            const FlagState sr = (idn[1] > 0 && idn[1] <= m.ncell) ? flag_state(m, idn[1], gamma) : sc;
+            if (ed > 0.0 && err_grad(sl.d, sc.d, sr.d, 1e-10) > ed) ok = true;
+            else if (ep > 0.0 && err_grad(sl.p, sc.p, sr.p, 1e-10) > ep) ok = true;
+            else if (ev > 0.0 &&
+                     err_grad_u(sl.v[d], sc.v[d], sr.v[d], sl.p, sc.p, sr.p, sl.d, sc.d, sr.d, gamma) > ev)
+                ok = true;
         }
-        const FlagState sl = (idn[0] > 0 && idn[0] <= m.ncell) ? flag_state(m, idn[0], gamma) : sc;
-        const FlagState sr = (idn[1] > 0 && idn[1] <= m.ncell) ? flag_state(m, idn[1], gamma) : sc;
-        if (ed > 0.0 && err_grad(sl.d, sc.d, sr.d, 1e-10) > ed) ok = true;
-        else if (ep > 0.0 && err_grad(sl.p, sc.p, sr.p, 1e-10) > ep) ok = true;
-        else if (ev > 0.0 && err_grad_u(sl.v[d], sc.v[d], sr.v[d], sl.p, sc.p, sr.p, sl.d, sc.d, sr.d, gamma) > ev)
-            ok = true;
     }
-    out[k] = ok ? 1 : 0;
+    const unsigned long long b = __ballot(ok);
+    const int lane = threadIdx.x & 63;
+    if (valid && ic0 == 0) bits[p] = (unsigned char)(bits[p] | ((b >> (lane & 56)) & 0xFFull));
+}
+
+// One smooth_fine pass (TreeUpdater.cpp:217-268) over a level list: every
+// unflagged cell with a flagged face neighbour at the same level (its
+// sibling, or the cell of the neighbour oct on that side when that oct
+// exists) becomes flagged, all decided from the flags before the pass
+// (`in`), written to `out`.  One thread per oct.
+__global__ __launch_bounds__(256) void k_smooth_bits(LevelDev lv, const unsigned char* __restrict__ in,
+                                                     unsigned char* __restrict__ out) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= lv.noct) return;
+    const int b = in[p];
+    int nb[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const int E = lv.nbo[(size_t)f * lv.noct + p];
+        nb[f] = E >= 0 ? in[E] : 0;
+    }
+    int r = b;
+#pragma unroll
+    for (int ic0 = 0; ic0 < 8; ++ic0) {
+        bool any = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int icS = ic0 ^ (1 << d);
+            const int par = (ic0 >> d) & 1;
+            // Sibling on the side opposite to the cell's parity, the
+            // neighbour oct's cell icS on the cell's own side.
+            any = any || ((b >> icS) & 1) || ((nb[2 * d + par] >> icS) & 1);
+        }
+        if (!((b >> ic0) & 1) && any) r |= 1 << ic0;
+    }
+    out[p] = (unsigned char)r;
 }
 
 // One level of a snapshot's hydro file (RamsesWriter.cpp:163-233), as the
@@ -2182,12 +2261,23 @@ hipError_t launch_set_uold_restrict(const MeshDev& m, const LevelDev& lv, const 
     return hipGetLastError();
 }
 
-hipError_t launch_grad_flags(const MeshDev& m, const LevelDev& lv, const int* d_nbor, double ed, double ep,
-                             double ev, double gamma, unsigned char* d_out, hipStream_t stream) {
+hipError_t launch_flag_bits(const MeshDev& m, const LevelDev& lv, const int* d_nbor, double ed, double ep,
+                            double ev, double gamma, int nexpand, unsigned char* d_a, unsigned char* d_b,
+                            unsigned char** d_result, hipStream_t stream) {
+    *d_result = d_a;
     if (lv.noct == 0) return hipSuccess;
-    const int n = lv.noct * 8;
-    hipLaunchKernelGGL(k_grad_flags, dim3((n + 255) / 256), dim3(256), 0, stream, m, lv, d_nbor, ed, ep, ev, gamma,
-                       d_out);
+    const dim3 go((lv.noct + 255) / 256), gc((lv.noct * 8 + 255) / 256), blk(256);
+    unsigned char* cur = d_a;
+    unsigned char* oth = d_b;
+    hipLaunchKernelGGL(k_smooth_bits, go, blk, 0, stream, lv, (const unsigned char*)cur, oth);
+    std::swap(cur, oth);
+    if (ed > 0.0 || ep > 0.0 || ev > 0.0)
+        hipLaunchKernelGGL(k_grad_bits, gc, blk, 0, stream, m, lv, d_nbor, ed, ep, ev, gamma, cur);
+    for (int i = 0; i < nexpand; ++i) {
+        hipLaunchKernelGGL(k_smooth_bits, go, blk, 0, stream, lv, (const unsigned char*)cur, oth);
+        std::swap(cur, oth);
+    }
+    *d_result = cur;
     return hipGetLastError();
 }
 

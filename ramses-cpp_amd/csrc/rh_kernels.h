@@ -130,9 +130,8 @@ hipError_t launch_fill_keys(unsigned long long* keys, int n, double value, hipSt
 hipError_t launch_keys_to_dt(const unsigned long long* keys, int n, double* dt, hipStream_t stream);
 // Adaptive run: prolongation into new octs (d_cells: 8 ints per new oct,
 // the refined cell, its six neighbours, the new grid; the same values go to
-// the new grid's entries of d_father [ngridmax] and d_nbor [6*ngridmax]),
-// restriction of a list into its fathers, and the gradient refinement
-// criteria of one list (d_out[8*pos + ic-1]).
+// the new grid's entries of d_father [ngridmax] and d_nbor [6*ngridmax]) and
+// restriction of a list into its fathers.
 hipError_t launch_new_octs(const MeshDev& m, int n, const int* d_cells, const GasParams& gp, int interpol_var,
                            int* d_father, int* d_nbor, hipStream_t stream);
 hipError_t launch_restrict(const MeshDev& m, const LevelDev& fine, const int* d_father, hipStream_t stream);
@@ -154,8 +153,14 @@ constexpr int AMR_T_SIZE = 3 * AMR_T_LEVELS + 2;
 hipError_t launch_amr_times(double* d_T, int op, int il, int lmin, double nsub_prev, double nsub,
                             hipStream_t stream);
 
-hipError_t launch_grad_flags(const MeshDev& m, const LevelDev& lv, const int* d_nbor, double ed, double ep,
-                             double ev, double gamma, unsigned char* d_out, hipStream_t stream);
+// flag_fine's passes over a level list on the device (flags: one byte per
+// oct in list order, bit ic-1 for cell ic): one smooth_fine pass, the
+// gradient criteria OR-ed in (when a threshold is positive), then nexpand
+// smooth_fine passes.  d_a holds the initial flags; d_a and d_b alternate
+// between the passes, and *d_result names the one holding the final flags.
+hipError_t launch_flag_bits(const MeshDev& m, const LevelDev& lv, const int* d_nbor, double ed, double ep,
+                            double ev, double gamma, int nexpand, unsigned char* d_a, unsigned char* d_b,
+                            unsigned char** d_result, hipStream_t stream);
 hipError_t launch_interpol(int n_octs, const double* d_u1, double* d_u2, const GasParams& gp, int interpol_var,
                            hipStream_t stream);
 // Snapshot hydro records of one level list (n grids, d_grids in list order)
