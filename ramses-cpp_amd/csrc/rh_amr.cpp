@@ -3,9 +3,10 @@
 //
 // The host owns the tree (AmrGrid's integer arrays, ncpu = 1) and edits it as
 // the reference's tree updater does; every operation on the state runs on the
-// device: the hydro step through the context's kernels, the gradient
-// refinement criteria (k_grad_flags: one byte per cell comes back), the
-// prolongation into new octs (k_new_octs) and the restriction (k_restrict).
+// device: the hydro step through the context's kernels, flag_fine's passes
+// over a level list (smoothing and gradient criteria, k_smooth_bits and
+// k_grad_bits: one byte per oct goes each way), the prolongation into new
+// octs (k_new_octs) and the restriction (k_restrict).
 // After a tree edit the context's mesh epoch moves, so its level tables are
 // rebuilt on next use, and the father/nbor arrays the AMR kernels read are
 // uploaded again.
@@ -72,7 +73,12 @@ struct rh_amr {
     int* d_nbor = nullptr;
     int* d_cells = nullptr;
     int ncells_cap = 0;
-    unsigned char* d_flags = nullptr;
+    // Pinned staging of the new octs' index lists (ncells_cap * 8 ints); the
+    // next make_grid_fine waits on cells_staged before refilling it.
+    int* h_cells = nullptr;
+    hipEvent_t cells_staged = nullptr;
+    unsigned char* d_flags = nullptr;  // 8*ngridmax bytes: two flag buffers of flag_fine
+    unsigned char* h_flags = nullptr;  // ngridmax bytes of pinned host memory: a list's flags
     double* d_times = nullptr;  // rh::AMR_T_SIZE doubles
     // Each level list's grids in list order, and the edit count of the list
     // they were read at (list_version is bumped by every edit of the list).
@@ -632,51 +638,59 @@ int coarse_gradients(rh_amr* a) {
     return RH_OK;
 }
 
-// Gradient criteria of a level list on the device; flags come back per cell.
-int list_gradients(rh_amr* a, int il, const std::vector<int>& grids) {
-    const rh::LevelDev* lv;
-    int st = get_level(a->ctx, il, &lv);
-    if (st) return st;
-    if (lv->noct == 0) return RH_OK;
-    RH_ACHK(upload_topology(a));
-    RH_AHIP(a, rh::launch_grad_flags(mesh_dev(a->ctx), *lv, a->d_nbor, a->ap.err_grad_d, a->ap.err_grad_p,
-                                     a->ap.err_grad_v, a->gamma, a->d_flags, a->ctx->stream));
-    std::vector<unsigned char> fl((size_t)lv->noct * 8);
-    RH_AHIP(a, hipMemcpyAsync(fl.data(), a->d_flags, fl.size(), hipMemcpyDeviceToHost, a->ctx->stream));
-    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
-    parallel_for((int)grids.size(), [&](int pos) {
-        for (int ic = 1; ic <= 8; ++ic) {
-            int idc = cell(a, grids[pos], ic) - 1;
-            if (a->flag1[idc] == 0 && fl[(size_t)pos * 8 + ic - 1]) a->flag1[idc] = 1;
-        }
-    });
-    return RH_OK;
-}
-
-// TreeUpdater.cpp:328-583
+// TreeUpdater.cpp:328-583.  The coarse level (il = 1) is flagged on the
+// host.  For a level list the host sets the initial flags (every cell below
+// levelmin, else the cells whose son has a refined or flagged cell) and the
+// device runs the passes over the list -- smooth_fine, the gradient
+// criteria, nexpand more smooth_fine passes (launch_flag_bits) -- with one
+// byte of flags per oct going each way; ensure_ref_rules, which reads the
+// tree, stays on the host.  test_flag (applied when il > levelmin and icount
+// < nsubcycle) would flag the cells whose son has a refined or flagged cell:
+// the initial flags already hold them and flags only grow in between, so it
+// changes nothing and is not run.
 int flag_fine(rh_amr* a, int il, int icount, int nsub) {
     if (il >= a->nlevelmax) return RH_OK;
     const int lmin = a->levelmin;
-    // The list does not change during the passes (they only set flags).
-    static const std::vector<int> no_grids;
-    const std::vector<int>& grids = il == 1 ? no_grids : list_grids(a, il);
     if (il == 1) {
         for (int i = 0; i < a->ncoarse; ++i) a->flag1[i] = 0;
         for (int i = 1; i <= a->ncoarse; ++i)
             if (il < lmin || child_active(a, i - 1)) a->flag1[i - 1] = 1;
-    } else {
-        parallel_for((int)grids.size(), [&](int k) {
-            for (int ic = 1; ic <= 8; ++ic) {
-                int idc = cell(a, grids[k], ic) - 1;
-                a->flag1[idc] = (il < lmin || child_active(a, idc)) ? 1 : 0;
-            }
-        });
+        static const std::vector<int> no_grids;
+        if (il >= lmin) smooth_fine(a, il, no_grids);
+        if (il >= lmin && (a->ap.err_grad_d > 0.0 || a->ap.err_grad_p > 0.0 || a->ap.err_grad_v > 0.0))
+            RH_ACHK(coarse_gradients(a));
+        if (il > lmin && icount < nsub) test_flag(a, il, no_grids);
+        for (int i = 0; i < a->ap.nexpand; ++i) smooth_fine(a, il, no_grids);
+        return RH_OK;
     }
-    if (il >= lmin) smooth_fine(a, il, grids);
-    if (il >= lmin && (a->ap.err_grad_d > 0.0 || a->ap.err_grad_p > 0.0 || a->ap.err_grad_v > 0.0))
-        RH_ACHK(il == 1 ? coarse_gradients(a) : list_gradients(a, il, grids));
-    if (il > lmin && icount < nsub) test_flag(a, il, grids);
-    for (int i = 0; i < a->ap.nexpand; ++i) smooth_fine(a, il, grids);
+    // The list does not change during the passes (they only set flags).
+    const std::vector<int>& grids = list_grids(a, il);
+    const int n = (int)grids.size();
+    unsigned char* hf = a->h_flags;
+    parallel_for(n, [&](int k) {
+        unsigned char bits = 0;
+        for (int ic = 1; ic <= 8; ++ic)
+            if (il < lmin || child_active(a, cell(a, grids[k], ic) - 1)) bits |= (unsigned char)(1u << (ic - 1));
+        hf[k] = bits;
+    });
+    // Below levelmin every cell is flagged and the passes change nothing.
+    if (il >= lmin && n > 0) {
+        const rh::LevelDev* lv;
+        RH_ACHK(get_level(a->ctx, il, &lv));
+        if (lv->noct != n) return afail(a, RH_ERR_HOST, "level list and level tables disagree");
+        RH_ACHK(upload_topology(a));
+        const hipStream_t stream = a->ctx->stream;
+        unsigned char* res = nullptr;
+        RH_AHIP(a, hipMemcpyAsync(a->d_flags, hf, (size_t)n, hipMemcpyHostToDevice, stream));
+        RH_AHIP(a, rh::launch_flag_bits(mesh_dev(a->ctx), *lv, a->d_nbor, a->ap.err_grad_d, a->ap.err_grad_p,
+                                        a->ap.err_grad_v, a->gamma, a->ap.nexpand, a->d_flags,
+                                        a->d_flags + a->ngridmax, &res, stream));
+        RH_AHIP(a, hipMemcpyAsync(hf, res, (size_t)n, hipMemcpyDeviceToHost, stream));
+        RH_AHIP(a, hipStreamSynchronize(stream));
+    }
+    parallel_for(n, [&](int k) {
+        for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, grids[k], ic) - 1] = (hf[k] >> (ic - 1)) & 1;
+    });
     if (il > lmin && icount < nsub) ensure_ref_rules(a, grids);
     return RH_OK;
 }
@@ -744,18 +758,28 @@ int make_grid_fine(rh_amr* a, int il) {
         add_to_list(a, new_ig, il + 1);
     }
     const int n = (int)cells.size();
+    if (!a->cells_staged) RH_AHIP(a, hipEventCreateWithFlags(&a->cells_staged, hipEventDisableTiming));
+    // The previous copy out of the staging buffer has finished (and, when the
+    // buffers grow, every kernel that reads d_cells).
+    RH_AHIP(a, hipEventSynchronize(a->cells_staged));
     if (n > a->ncells_cap) {
+        RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));
         if (a->d_cells) (void)hipFree(a->d_cells);
+        if (a->h_cells) (void)hipHostFree(a->h_cells);
         a->d_cells = nullptr;
+        a->h_cells = nullptr;
         a->ncells_cap = 0;
-        RH_AHIP(a, hipMalloc(&a->d_cells, (size_t)n * 8 * sizeof(int)));
-        a->ncells_cap = n;
+        const int cap = n + n / 4 + 64;
+        RH_AHIP(a, hipMalloc(&a->d_cells, (size_t)cap * 8 * sizeof(int)));
+        RH_AHIP(a, hipHostMalloc((void**)&a->h_cells, (size_t)cap * 8 * sizeof(int), hipHostMallocDefault));
+        a->ncells_cap = cap;
     }
-    RH_AHIP(a, hipMemcpyAsync(a->d_cells, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice,
+    std::memcpy(a->h_cells, idx.data(), idx.size() * sizeof(int));
+    RH_AHIP(a, hipMemcpyAsync(a->d_cells, a->h_cells, idx.size() * sizeof(int), hipMemcpyHostToDevice,
                               a->ctx->stream));
+    RH_AHIP(a, hipEventRecord(a->cells_staged, a->ctx->stream));
     RH_AHIP(a, rh::launch_new_octs(mesh_dev(a->ctx), n, a->d_cells, gas(a->ctx, a->gamma), a->hp.interpol_var,
                                    a->d_father, a->d_nbor, a->ctx->stream));
-    RH_AHIP(a, hipStreamSynchronize(a->ctx->stream));  // idx goes out of scope
     tree_changed(a, il + 1);
     return RH_OK;
 }
@@ -973,8 +997,12 @@ int step_coarse(rh_amr* a) {
 void destroy(rh_amr* a) {
     if (a->ctx) {
         (void)hipSetDevice(a->hp.device);
+        (void)hipStreamSynchronize(a->ctx->stream);
         for (void* p : {(void*)a->d_father, (void*)a->d_nbor, (void*)a->d_cells, (void*)a->d_flags, (void*)a->d_times})
             if (p) (void)hipFree(p);
+        if (a->h_flags) (void)hipHostFree(a->h_flags);
+        if (a->h_cells) (void)hipHostFree(a->h_cells);
+        if (a->cells_staged) (void)hipEventDestroy(a->cells_staged);
         rh_destroy(a->ctx);
     }
     if (a->pool_user) pool_release();
@@ -1033,6 +1061,7 @@ int rh_amr_create(const rh_params* hydro, const rh_amr_params* ap, rh_amr** out)
             hipError_t e = hipMalloc(&a->d_father, (size_t)a->ngridmax * sizeof(int));
             if (e == hipSuccess) e = hipMalloc(&a->d_nbor, (size_t)6 * a->ngridmax * sizeof(int));
             if (e == hipSuccess) e = hipMalloc(&a->d_flags, (size_t)8 * a->ngridmax);
+            if (e == hipSuccess) e = hipHostMalloc((void**)&a->h_flags, (size_t)a->ngridmax, hipHostMallocDefault);
             if (e == hipSuccess) e = hipMalloc(&a->d_times, (size_t)rh::AMR_T_SIZE * sizeof(double));
             if (e == hipSuccess) {
                 const std::vector<double> zero(rh::AMR_T_SIZE, 0.0);  // dtnew = dtold = 0, t = 0
