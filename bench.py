@@ -19,7 +19,7 @@ import platform
 import subprocess
 import sys
 import time
-from ctypes import byref, c_double as C_double, c_longlong as C_longlong
+from ctypes import byref, c_double as C_double, c_int as C_int, c_longlong as C_longlong
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ramses-cpp_amd"))
@@ -31,17 +31,26 @@ BYTES_PER_CELL_UPDATE = 80  # read uold (5 x FP64) + write unew (5 x FP64)
 # Counter figures of the Godunov kernel from rocprofv3 runs of this script
 # (profiles/), keyed by (level, riemann, slope_type, kernel variant, form):
 #   traffic  HBM bytes per launch, from --pmc FETCH_SIZE and --pmc WRITE_SIZE
-#            runs, each counter scaled by what it reads for k_set_unew, a copy
-#            of known size, in the same run;
+#            runs, each counter scaled by the ratio it reads for a copy of
+#            known size with the same access granularity (x1.3965 for reads,
+#            x0.9143 for writes);
 #   busy     VALU utilisation (SQ_ACTIVE_INST_VALU x 4 / SIMDs / busy cycles);
 #   insts_per_cell  VALU instructions per wave = per cell (one cell per lane),
-#            SQ_INSTS_VALU / SQ_WAVES.
+#            SQ_INSTS_VALU / SQ_WAVES, of a launch with or without the next
+#            step's Courant candidates (courant_epilogue).
 # Counters cannot be read while timing.  Each entry records the hash of the
 # kernel sources it was measured on (KERNEL_SOURCES); a build whose sources
 # hash differently reports null instead of a stale figure.
 KERNEL_SOURCES = ["ramses-cpp_amd/csrc/rh_hydro.hip", "ramses-cpp_amd/csrc/rh_physics.h",
                   "ramses-cpp_amd/csrc/rh_kernels.h", "ramses-cpp_amd/Makefile"]
-PROFILED = {}
+PROFILED = {
+    (9, "hllc", 1, 2, "fp64"): {"sources": "790f52b31b5c020a", "traffic": 1665454886,
+                                "valu": {"busy": 0.763, "insts_per_cell": 5985.4, "courant_epilogue": True}},
+    (9, "hllc", 1, 2, "mixed"): {"sources": "790f52b31b5c020a", "traffic": 1684012336,
+                                 "valu": {"busy": 0.846, "insts_per_cell": 3784.7, "courant_epilogue": False}},
+    (9, "hllc", 1, 1, "mixed"): {"sources": "790f52b31b5c020a", "traffic": None,
+                                 "valu": {"busy": 0.836, "insts_per_cell": 4003.6, "courant_epilogue": True}},
+}
 
 
 def kernel_source_hash() -> str:
@@ -242,6 +251,18 @@ def measure_amr(args, rhydro, np, which, precision=None):
     # run has them off and a second, identical run reads the Godunov time.
     timed_run = which != "sedov"
     ctx = run.solver_context()
+    tune = None
+    if which != "sedov":
+        # Godunov kernel variant picked on the finest level (the bulk of the
+        # cell-updates), after the initial refinement.
+        lmax = kw["levelmax"]
+        dxl = 1.0 / 2 ** lmax
+        dt = C_double()
+        if L.rh_courant(ctx, lmax, dxl, 1.4, 0.8, byref(dt)) == 0:
+            bv = C_int()
+            msv = (C_double * rhydro.RH_KERNEL_VARIANTS)()
+            if L.rh_autotune(ctx, lmax, dt.value, dxl, 2, byref(bv), msv) == 0:
+                tune = {"level": lmax, "chosen_variant": bv.value, "godunov_ms_by_variant": list(msv)}
     L.rh_timers_enable(ctx, 1 if timed_run else 0)
     L.rh_timers_reset(ctx)
     _, _, cu0 = run.stats()
@@ -266,7 +287,7 @@ def measure_amr(args, rhydro, np, which, precision=None):
     out = {"workload": name, "precision": "mixed" if precision == rhydro.RH_PREC_MIXED else "fp64",
            "coarse_steps": steps, "fine_steps": nstep, "cell_updates": int(cu1 - cu0),
            "value": (cu1 - cu0) / wall, "wall_s": wall, "init_s": t_init, "godunov_ms_total": ms.value,
-           "godunov_launches": n.value, "octs_per_level": st["numbl"].tolist(),
+           "godunov_launches": n.value, "octs_per_level": st["numbl"].tolist(), "autotune": tune,
            "host_phase_ms": {k: ph1[k] - ph0[k] for k in ph1}, "_state": st}
     print(f"[bench] amr {which}: {steps} coarse steps, {cu1 - cu0} cell-updates in {wall:.3f}s "
           f"(godunov {ms.value:.1f} ms), lists {st['numbl'].tolist()}", file=sys.stderr, flush=True)
@@ -445,7 +466,7 @@ def main() -> int:
     ap.add_argument("--only", choices=["ensemble", "amr_sedov", "amr_kh", "sedov64"],
                     help="run one secondary workload alone and print its record (for profiling)")
     ap.add_argument("--amr-steps", type=int, default=50, help="coarse steps of the AMR Sedov run (0: skip)")
-    ap.add_argument("--kh-steps", type=int, default=1, help="coarse steps of the AMR Kelvin-Helmholtz run (0: skip)")
+    ap.add_argument("--kh-steps", type=int, default=5, help="coarse steps of the AMR Kelvin-Helmholtz run (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
