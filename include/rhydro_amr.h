@@ -72,8 +72,10 @@ rh_ctx* rh_amr_context(rh_amr* run);
 /* Host wall time (ms, accumulated since creation) of the phases of the
  * coarse steps: ms[0] hydro calls (Courant, set_unew, godunov_fine, set_uold,
  * each waiting for its kernels), ms[1] flag_fine, ms[2] make_grid_fine,
- * ms[3] remove_grid_fine, ms[4] restrict_fine, and ms[5] the part of all
- * phases (and of creation) spent rebuilding level tables after tree edits. */
+ * ms[3] remove_grid_fine, ms[4] restrict_fine, ms[5] the part of all
+ * phases (and of creation) spent rebuilding level tables after tree edits,
+ * and ms[6] the part spent waiting for the stream before the flags' copy
+ * back is read (which includes every launch queued before it). */
 int rh_amr_timers(rh_amr* run, double* ms);
 
 #ifdef __cplusplus

@@ -93,6 +93,16 @@ int rh_get_kernel_variant(rh_ctx* ctx, int* variant);
  * restored after every trial. */
 int rh_autotune(rh_ctx* ctx, int ilevel, double dt, double dx, int reps, int* best_variant, double* ms_per_variant);
 
+/* Multi-step calls whose steps run with uold and unew exchanged (levels of
+ * leaf cells with same-level neighbours only: rh_run_uniform, and
+ * rh_run_ensemble on periodic boxes) replay their launch sequence from a HIP
+ * graph captured on first use and kept while the level tables, state
+ * buffers, parameters and step count stay the same (on by default; eager
+ * launches while the timers are on).  rh_graph_stats reports how many graphs
+ * the context captured and launched. */
+int rh_set_graphs(rh_ctx* ctx, int enable);
+int rh_graph_stats(rh_ctx* ctx, long long* captured, long long* launched);
+
 /* Leaf cells of a level in the bound mesh (cell-updates per godunov call). */
 int rh_level_leaf_cells(rh_ctx* ctx, int ilevel, long long* nleaf);
 

@@ -91,6 +91,9 @@ struct rh_amr {
     // Host wall time per phase of the coarse steps (ms): hydro calls, flag_fine,
     // make_grid_fine, remove_grid_fine, restrict_fine.
     double phase_ms[5] = {0, 0, 0, 0, 0};
+    // The part of the phases spent waiting for the stream (the flags' copy
+    // back, which also waits for every launch queued before it).
+    double wait_ms = 0.0;
 };
 
 using namespace rhi;
@@ -686,7 +689,9 @@ int flag_fine(rh_amr* a, int il, int icount, int nsub) {
                                         a->ap.err_grad_v, a->gamma, a->ap.nexpand, a->d_flags,
                                         a->d_flags + a->ngridmax, &res, stream));
         RH_AHIP(a, hipMemcpyAsync(hf, res, (size_t)n, hipMemcpyDeviceToHost, stream));
+        const auto w0 = std::chrono::steady_clock::now();
         RH_AHIP(a, hipStreamSynchronize(stream));
+        a->wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     }
     parallel_for(n, [&](int k) {
         for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, grids[k], ic) - 1] = (hf[k] >> (ic - 1)) & 1;
@@ -1174,6 +1179,7 @@ int rh_amr_timers(rh_amr* a, double* ms) {
         if (!a || !ms) return RH_ERR_ARG;
         for (int k = 0; k < 5; ++k) ms[k] = a->phase_ms[k];
         ms[5] = a->ctx->table_ms;
+        ms[6] = a->wait_ms;
         return RH_OK;
     });
 }

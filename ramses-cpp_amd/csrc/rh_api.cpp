@@ -286,8 +286,30 @@ double exact_inverse(double dx) {
     return 1.0 / dx;
 }
 
+void drop_graph(StepGraph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.key.clear();
+}
+
+void key_add(GraphKey& k, const rh::LevelDev& lv) {
+    k.add(lv.noct).add(lv.nrec).add(lv.nslots).add(lv.ntgt).add(lv.nleaf).add(lv.grid).add(lv.leaf).add(lv.nbo);
+    k.add(lv.rec).add(lv.rec_slot).add(lv.tgt_cell).add(lv.tgt_off).add(lv.tgt_slot).add(lv.slot_r);
+    k.add(lv.nbox).add(lv.obox).add(lv.box_len);
+}
+
+void key_add(GraphKey& k, const rh::MeshDev& m) {
+    k.add(m.ncoarse).add(m.ngridmax).add(m.ncell).add(m.nx).add(m.ny).add(m.nz).add(m.uold).add(m.unew).add(m.f);
+}
+
+void key_add(GraphKey& k, const rh::GasParams& gp) {
+    k.add(gp.gamma).add(gp.gm1).add(gp.smalle).add(gp.smallp).add(gp.slope_type).add(gp.interpol_type);
+}
+
 void unbind(rh_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
+    drop_graph(c->g_uniform);
+    drop_graph(c->g_ensemble);
     for (auto& L : c->levels) free_level(L);
     free_level(c->ens_level);
     for (void* p : {(void*)c->d_uold, (void*)c->d_unew, (void*)c->d_f, (void*)c->d_boxlen, (void*)c->d_keys})
@@ -660,40 +682,53 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
         // cells of both buffers hold the final state, as after set_uold.
         const bool fused = c->variant != 0 && lv->ntgt == 0 && lv->nrec == 0 && lv->nleaf == 8LL * lv->noct &&
                            lv->noct > 0;
-        if (fused) {
+        if (fused && nsteps > 0) {
             const int nb = rh::godunov_blocks(*lv, c->variant);
-            if ((st = ensure_part(c, nb + 1))) return st;
-            rh::MeshDev mk = m;
-            for (int k = 0; k < nsteps; ++k) {
-                if (k == 0 && (st = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt))) return st;
-                const bool next = k + 1 < nsteps;
-                rh::FusedCourant fc;
-                fc.pre_from_uold = true;
-                fc.part = next ? c->d_part : nullptr;
-                fc.courant_factor = courant_factor;
-                {
-                    Timed tm(c, TK_GODUNOV);
-                    hipError_t e = rh::launch_godunov(mk, *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision,
-                                                      c->variant, c->d_dt + k, dx, exact_inverse(dx), c->stream, fc);
-                    if (e == hipErrorNotSupported)
-                        return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
-                    if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
-                    tm.done();
+            // Both sizes up front: the launches may be captured into a graph,
+            // where no allocation may happen.
+            if ((st = ensure_part(c, std::max(nb, rh::courant_blocks(*lv)) + 1))) return st;
+            const rh::GasParams gp = gas(c, c->p.gamma);
+            const double inv_dx = exact_inverse(dx), dt_init = c->p.boxlen / 1e-10;
+            auto issue = [&]() -> int {
+                int s2;
+                rh::MeshDev mk = m;
+                for (int k = 0; k < nsteps; ++k) {
+                    if (k == 0 && (s2 = do_courant(c, lv, dx, c->p.gamma, courant_factor, c->d_dt))) return s2;
+                    const bool next = k + 1 < nsteps;
+                    rh::FusedCourant fc;
+                    fc.pre_from_uold = true;
+                    fc.part = next ? c->d_part : nullptr;
+                    fc.courant_factor = courant_factor;
+                    {
+                        Timed tm(c, TK_GODUNOV);
+                        hipError_t e = rh::launch_godunov(mk, *lv, gp, c->p.riemann, c->p.precision, c->variant,
+                                                          c->d_dt + k, dx, inv_dx, c->stream, fc);
+                        if (e == hipErrorNotSupported)
+                            return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
+                        if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
+                        tm.done();
+                    }
+                    if (next) {
+                        Timed tm(c, TK_COURANT);
+                        RH_HIP(c, rh::launch_courant_final(c->d_part, nb, dt_init, c->d_dt + k + 1, c->stream));
+                        tm.done();
+                    }
+                    std::swap(mk.uold, mk.unew);
                 }
-                if (next) {
-                    Timed tm(c, TK_COURANT);
-                    RH_HIP(c, rh::launch_courant_final(c->d_part, nb, c->p.boxlen / 1e-10, c->d_dt + k + 1, c->stream));
-                    tm.done();
-                }
-                std::swap(mk.uold, mk.unew);
-            }
-            if (nsteps > 0) {
-                // mk.uold holds the final state; copy the level's cells into the
-                // other buffer (set_unew's copy, uold -> unew).
+                // mk.uold holds the final state; copy the level's cells into
+                // the other buffer (set_unew's copy, uold -> unew).
                 Timed tm(c, TK_SET_UOLD);
                 RH_HIP(c, rh::launch_set_unew(mk, *lv, c->stream));
                 tm.done();
-            }
+                return RH_OK;
+            };
+            GraphKey key;
+            key_add(key, *lv);
+            key_add(key, m);
+            key_add(key, gp);
+            key.add(c->p.riemann).add(c->p.precision).add(c->variant).add(dx).add(inv_dx).add(courant_factor);
+            key.add(dt_init).add(nsteps).add(nb).add(c->d_dt).add(c->d_part).add(c->npart);
+            if ((st = run_graph(c, c->g_uniform, key, issue))) return st;
         }
         // Otherwise step k is courant -> set_unew -> godunov_fine -> set_uold.
         // Between two steps, set_uold of the first and courant + set_unew of the
@@ -724,6 +759,29 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
                                      c->stream));
         RH_HIP(c, hipStreamSynchronize(c->stream));
         return post_call(c, RH_UOLD | RH_UNEW);
+    });
+}
+
+int rh_set_graphs(rh_ctx* c, int enable) {
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        c->graphs = enable != 0;
+        if (!c->graphs) {
+            (void)hipSetDevice(c->p.device);
+            (void)hipStreamSynchronize(c->stream);
+            drop_graph(c->g_uniform);
+            drop_graph(c->g_ensemble);
+        }
+        return RH_OK;
+    });
+}
+
+int rh_graph_stats(rh_ctx* c, long long* captured, long long* launched) {
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        if (!c) return RH_ERR_ARG;
+        if (captured) *captured = c->graph_captures;
+        if (launched) *launched = c->graph_launches;
+        return RH_OK;
     });
 }
 

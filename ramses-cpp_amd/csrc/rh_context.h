@@ -43,6 +43,24 @@ struct EnsBox {
     long long coff = 0, goff = 0;
 };
 
+// A multi-step launch sequence captured into a HIP graph, with the bytes of
+// every value its launches were captured with (kernel arguments, device
+// pointers, counts); it is replayed while a new call's values are the same.
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> key;
+};
+
+// Builds a StepGraph key field by field (no struct padding in it).
+struct GraphKey {
+    std::vector<unsigned char> b;
+    template <typename T> GraphKey& add(const T& v) {
+        const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+        b.insert(b.end(), p, p + sizeof(T));
+        return *this;
+    }
+};
+
 }  // namespace rhi
 
 struct rh_ctx {
@@ -90,6 +108,12 @@ struct rh_ctx {
     double* d_boxlen = nullptr;   // [2*nbox]: dx, then 1/dx (0 when not exact)
     unsigned long long* d_keys = nullptr;  // [nkeys]: ordered keys of the per-box dt
     int nkeys = 0;
+    // Multi-step runs with exchanged state buffers (rh_run_uniform,
+    // rh_run_ensemble) replay a captured graph of their launches unless
+    // graphs are off (rh_set_graphs) or the timers are on.
+    bool graphs = true;
+    rhi::StepGraph g_uniform, g_ensemble;
+    long long graph_captures = 0, graph_launches = 0;
 };
 
 namespace rhi {
@@ -129,6 +153,44 @@ int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx);
 int set_uold_restrict_dev(rh_ctx* c, int ilevel, const int* d_father);
 // Frees every device buffer of the current binding (single or ensemble).
 void unbind(rh_ctx* c);
+
+void drop_graph(StepGraph& g);
+// The launch-relevant values of a level's tables, a mesh and the gas
+// parameters, appended to a graph key.
+void key_add(GraphKey& k, const rh::LevelDev& lv);
+void key_add(GraphKey& k, const rh::MeshDev& m);
+void key_add(GraphKey& k, const rh::GasParams& gp);
+// Runs `issue` -- which enqueues a fixed launch sequence on c->stream and
+// returns an RH_* status -- through the graph g: captured from it when g is
+// empty or was captured with another key, then launched.  Eager (issue
+// runs as is) when graphs are off or the timers are on.  issue must not
+// allocate, free, synchronise or copy from pageable memory.
+template <class F> int run_graph(rh_ctx* c, StepGraph& g, const GraphKey& key, F&& issue) {
+    if (!c->graphs || c->timers) return issue();
+    if (!g.exec || g.key != key.b) {
+        // A launch of the old graph may still be queued.
+        if (g.exec) RH_HIP(c, hipStreamSynchronize(c->stream));
+        drop_graph(g);
+        RH_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int st = issue();
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (st != RH_OK || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st != RH_OK ? st : hip_fail(c, e, "hipStreamEndCapture");
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return hip_fail(c, ei, "hipGraphInstantiate");
+        g.exec = exec;
+        g.key = key.b;
+        ++c->graph_captures;
+    }
+    RH_HIP(c, hipGraphLaunch(g.exec, c->stream));
+    ++c->graph_launches;
+    return RH_OK;
+}
 
 // Allocates a device copy of src, owned by L (freed by free_level).
 template <typename T> hipError_t upload(DevLevel& L, T*& dst, const std::vector<T>& src, hipStream_t s) {

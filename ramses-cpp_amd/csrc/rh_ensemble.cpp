@@ -293,50 +293,61 @@ int rh_run_ensemble(rh_ctx* c, const int* levels, const double* dx, double coura
         const rh::MeshDev m = mesh_dev(c);
         const rh::GasParams gp = gas(c, c->p.gamma);
         const double dt_init = c->p.boxlen / 1e-10;
-        RH_HIP(c, rh::launch_fill_keys(c->d_keys, nk, dt_init, c->stream));
         // As in rh_run_uniform: boxes whose levels hold only leaf cells with
         // same-level neighbours step with uold and unew exchanged, each Godunov
         // launch taking the next step's per-box Courant keys from the values it
-        // writes; a final copy leaves the last state in both buffers.
+        // writes; a final copy leaves the last state in both buffers.  Those
+        // launches are replayed from a captured graph (run_graph).
         const bool exchanged = c->variant != 0 && lv.ntgt == 0 && lv.nrec == 0 && lv.nleaf == 8LL * lv.noct &&
                                lv.noct > 0;
-        if (exchanged) {
-            rh::MeshDev mk = m;
-            for (int k = 0; k < nsteps; ++k) {
-                unsigned long long* keys = c->d_keys + (size_t)k * nbox;
-                double* dts = c->d_dt + (size_t)k * nbox;
-                const bool next = k + 1 < nsteps;
-                if (k == 0) {
-                    Timed tm(c, TK_COURANT);
-                    RH_HIP(c, rh::launch_courant_boxes(mk, lv, c->p.gamma, courant_factor, keys, c->stream));
-                    RH_HIP(c, rh::launch_keys_to_dt(keys, nbox, dts, c->stream));
-                    tm.done();
+        if (exchanged && nsteps > 0) {
+            auto issue = [&]() -> int {
+                RH_HIP(c, rh::launch_fill_keys(c->d_keys, nk, dt_init, c->stream));
+                rh::MeshDev mk = m;
+                for (int k = 0; k < nsteps; ++k) {
+                    unsigned long long* keys = c->d_keys + (size_t)k * nbox;
+                    double* dts = c->d_dt + (size_t)k * nbox;
+                    const bool next = k + 1 < nsteps;
+                    if (k == 0) {
+                        Timed tm(c, TK_COURANT);
+                        RH_HIP(c, rh::launch_courant_boxes(mk, lv, c->p.gamma, courant_factor, keys, c->stream));
+                        RH_HIP(c, rh::launch_keys_to_dt(keys, nbox, dts, c->stream));
+                        tm.done();
+                    }
+                    rh::FusedCourant fc;
+                    fc.pre_from_uold = true;
+                    fc.keys = next ? keys + nbox : nullptr;
+                    fc.courant_factor = courant_factor;
+                    {
+                        Timed tm(c, TK_GODUNOV);
+                        hipError_t e = rh::launch_godunov(mk, lv, gp, c->p.riemann, c->p.precision, c->variant, dts,
+                                                          0.0, all_exact ? 1.0 : 0.0, c->stream, fc);
+                        if (e == hipErrorNotSupported)
+                            return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
+                        if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
+                        tm.done();
+                    }
+                    if (next) {
+                        Timed tm(c, TK_COURANT);
+                        RH_HIP(c, rh::launch_keys_to_dt(keys + nbox, nbox, dts + nbox, c->stream));
+                        tm.done();
+                    }
+                    std::swap(mk.uold, mk.unew);
                 }
-                rh::FusedCourant fc;
-                fc.pre_from_uold = true;
-                fc.keys = next ? keys + nbox : nullptr;
-                fc.courant_factor = courant_factor;
-                {
-                    Timed tm(c, TK_GODUNOV);
-                    hipError_t e = rh::launch_godunov(mk, lv, gp, c->p.riemann, c->p.precision, c->variant, dts, 0.0,
-                                                      all_exact ? 1.0 : 0.0, c->stream, fc);
-                    if (e == hipErrorNotSupported)
-                        return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
-                    if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
-                    tm.done();
-                }
-                if (next) {
-                    Timed tm(c, TK_COURANT);
-                    RH_HIP(c, rh::launch_keys_to_dt(keys + nbox, nbox, dts + nbox, c->stream));
-                    tm.done();
-                }
-                std::swap(mk.uold, mk.unew);
-            }
-            if (nsteps > 0) {
                 Timed tm(c, TK_SET_UOLD);
                 RH_HIP(c, rh::launch_set_unew(mk, lv, c->stream));
                 tm.done();
-            }
+                return RH_OK;
+            };
+            GraphKey key;
+            key_add(key, lv);
+            key_add(key, m);
+            key_add(key, gp);
+            key.add(c->p.riemann).add(c->p.precision).add(c->variant).add(all_exact).add(courant_factor);
+            key.add(dt_init).add(nsteps).add(nbox).add(nk).add(c->d_keys).add(c->d_dt);
+            if ((st = run_graph(c, c->g_ensemble, key, issue))) return st;
+        } else {
+            RH_HIP(c, rh::launch_fill_keys(c->d_keys, nk, dt_init, c->stream));
         }
         // Otherwise set_uold of one step and the Courant step and set_unew of the
         // next run as one fused pass.

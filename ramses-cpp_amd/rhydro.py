@@ -146,6 +146,8 @@ def lib() -> C.CDLL:
         L.rh_set_kernel_variant.argtypes = [C.c_void_p, C.c_int]
         L.rh_get_kernel_variant.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.rh_autotune.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int), _DP]
+        L.rh_set_graphs.argtypes = [C.c_void_p, C.c_int]
+        L.rh_graph_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.rh_add_gravity_source_terms.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.rh_synchro_hydro_fine.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.rh_amr_create.argtypes = [C.POINTER(RhParams), C.POINTER(RhAmrParams), C.POINTER(C.c_void_p)]
@@ -178,6 +180,7 @@ EXPORTED = [
     "rh_bind_ensemble", "rh_run_ensemble", "rh_ensemble_leaf_cells", "rh_add_gravity_source_terms",
     "rh_synchro_hydro_fine", "rh_amr_create", "rh_amr_destroy", "rh_amr_step", "rh_amr_mesh", "rh_amr_stats",
     "rh_amr_context", "rh_amr_timers", "rh_dump_snapshot", "rh_write_hydro", "rh_write_amr", "rh_amr_dump",
+    "rh_set_graphs", "rh_graph_stats",
 ]
 
 
@@ -566,6 +569,17 @@ class Solver:
         self._chk(lib().rh_get_kernel_variant(self._ctx, C.byref(v)))
         return v.value
 
+    def graphs(self, enable=True):
+        """Replay multi-step exchanged-buffer runs from captured HIP graphs
+        (the default) or launch every kernel eagerly."""
+        self._chk(lib().rh_set_graphs(self._ctx, 1 if enable else 0))
+
+    def graph_stats(self) -> tuple[int, int]:
+        """(graphs captured, graph launches) of this context so far."""
+        a, b = C.c_longlong(), C.c_longlong()
+        self._chk(lib().rh_graph_stats(self._ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def leaf_cells(self, ilevel) -> int:
         n = C.c_longlong()
         self._chk(lib().rh_level_leaf_cells(self._ctx, ilevel, C.byref(n)))
@@ -654,9 +668,9 @@ class AmrRun:
                     t=t, nstep=nstep, cell_updates=cu)
 
     def phase_ms(self) -> dict:
-        ms = np.zeros(6, np.float64)
+        ms = np.zeros(7, np.float64)
         self._chk(lib().rh_amr_timers(self._h, _dp(ms)))
-        return dict(zip(("hydro", "flag", "make", "remove", "restrict", "tables"), ms.tolist()))
+        return dict(zip(("hydro", "flag", "make", "remove", "restrict", "tables", "flag_wait"), ms.tolist()))
 
     def solver_context(self):
         return lib().rh_amr_context(self._h)
