@@ -103,6 +103,13 @@ int rh_autotune(rh_ctx* ctx, int ilevel, double dt, double dx, int reps, int* be
 int rh_set_graphs(rh_ctx* ctx, int enable);
 int rh_graph_stats(rh_ctx* ctx, long long* captured, long long* launched);
 
+/* rh_set_unew's copy (uold -> unew on every cell of the level's octs) done
+ * with the Godunov kernel's memory access pattern -- the eight cells of an
+ * oct in eight adjacent lanes, eight octs per 64-lane workgroup -- so that
+ * the byte counters of a copy of known size calibrate the Godunov kernel's.
+ * Same result as rh_set_unew. */
+int rh_set_unew_oct_lanes(rh_ctx* ctx, int ilevel);
+
 /* Leaf cells of a level in the bound mesh (cell-updates per godunov call). */
 int rh_level_leaf_cells(rh_ctx* ctx, int ilevel, long long* nleaf);
 

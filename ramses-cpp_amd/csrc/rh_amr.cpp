@@ -966,7 +966,10 @@ int amr_step(rh_amr* a, int il, int icount) {
         long long nleaf = 0;
         RH_ACHK(rh_level_leaf_cells(a->ctx, il, &nleaf));
         a->cell_updates += nleaf;
-        RH_ACHK(godunov_dev(a->ctx, il, a->d_times + rh::AMR_T_USED + il, dx));
+        // Without finer octs no reflux reached this level's unew since its
+        // set_unew (in the Courant pass), so unew still equals uold there.
+        const bool no_finer = il >= a->nlevelmax || count(a, il + 1) == 0;
+        RH_ACHK(godunov_dev(a->ctx, il, a->d_times + rh::AMR_T_USED + il, dx, no_finer));
     }
     if (il < a->nlevelmax && count(a, il + 1) > 0) {
         // set_uold of this level and restrict_fine of the next one in one

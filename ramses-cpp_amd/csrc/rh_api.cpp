@@ -338,10 +338,12 @@ int do_courant(rh_ctx* c, const rh::LevelDev* lv, double dx, double gamma, doubl
     return RH_OK;
 }
 
-int do_godunov(rh_ctx* c, const rh::LevelDev* lv, const double* d_dt, double dx) {
+int do_godunov(rh_ctx* c, const rh::LevelDev* lv, const double* d_dt, double dx, bool pre_from_uold = false) {
     Timed tm(c, TK_GODUNOV);
+    rh::FusedCourant fc;
+    fc.pre_from_uold = pre_from_uold && c->variant != 0;
     hipError_t e = rh::launch_godunov(mesh_dev(c), *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision,
-                                      c->variant, d_dt, dx, exact_inverse(dx), c->stream);
+                                      c->variant, d_dt, dx, exact_inverse(dx), c->stream, fc);
     if (e == hipErrorNotSupported) return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
     if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
     tm.done();
@@ -363,14 +365,14 @@ int courant_dev(rh_ctx* c, int ilevel, double dx, double gamma, double courant_f
     return do_courant(c, lv, dx, gamma, courant_factor, d_out, set_unew, at);
 }
 
-int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx) {
+int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx, bool unew_is_uold) {
     int st = check_bound(c);
     if (st) return st;
     if (c->mode != RH_DEVICE_RESIDENT) return fail(c, RH_ERR_ARG, "device time steps need a device-resident mesh");
     if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
     const rh::LevelDev* lv;
     if ((st = get_level(c, ilevel, &lv))) return st;
-    return do_godunov(c, lv, d_dt, dx);
+    return do_godunov(c, lv, d_dt, dx, unew_is_uold);
 }
 
 int set_uold_restrict_dev(rh_ctx* c, int ilevel, const int* d_father) {
@@ -508,6 +510,21 @@ int rh_set_unew(rh_ctx* c, int ilevel) {
         if ((st = get_level(c, ilevel, &lv))) return st;
         Timed tm(c, TK_SET_UNEW);
         RH_HIP(c, rh::launch_set_unew(mesh_dev(c), *lv, c->stream));
+        tm.done();
+        return post_call(c, RH_UNEW);
+    });
+}
+
+int rh_set_unew_oct_lanes(rh_ctx* c, int ilevel) {
+    return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
+        int st = check_bound(c);
+        if (st) return st;
+        if (ilevel <= 0) return fail(c, RH_ERR_UNSUPPORTED, "level 0 (coarse cells) is not on the GPU path");
+        if ((st = pre_call(c))) return st;
+        const rh::LevelDev* lv;
+        if ((st = get_level(c, ilevel, &lv))) return st;
+        Timed tm(c, TK_SET_UNEW);
+        RH_HIP(c, rh::launch_set_unew_oct_lanes(mesh_dev(c), *lv, c->stream));
         tm.done();
         return post_call(c, RH_UNEW);
     });
@@ -741,7 +758,9 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
                 RH_HIP(c, rh::launch_set_unew(m, *lv, c->stream));
                 tm.done();
             }
-            if ((st = do_godunov(c, lv, c->d_dt + k, dx))) return st;
+            // set_unew, or the join pass, has just made unew equal uold on the
+            // level's cells.
+            if ((st = do_godunov(c, lv, c->d_dt + k, dx, true))) return st;
             if (k + 1 < nsteps) {
                 if ((st = ensure_part(c, rh::courant_blocks(*lv) + 1))) return st;
                 Timed tm(c, TK_STEP_JOIN);

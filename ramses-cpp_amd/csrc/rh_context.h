@@ -146,7 +146,10 @@ int pre_call(rh_ctx* c);
 // device pointers (device-resident single mesh; no host wait).
 int courant_dev(rh_ctx* c, int ilevel, double dx, double gamma, double courant_factor, double* d_out,
                 bool set_unew = false, const rh::AmrTimesArgs& at = rh::AmrTimesArgs{});
-int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx);
+// unew_is_uold: the caller knows that unew equals uold on every leaf cell of
+// the level (set_unew ran and nothing wrote unew since), so the kernel reads
+// the cells' pre-update state from uold, which it reads anyway.
+int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx, bool unew_is_uold = false);
 // rh_set_uold(ilevel) followed by the restriction of level ilevel+1 into
 // its father cells (d_father: father cell of every grid, device memory), in
 // one launch; device-resident meshes only.

@@ -366,9 +366,12 @@ int rh_run_ensemble(rh_ctx* c, const int* levels, const double* dx, double coura
                 tm.done();
             }
             {
+                // set_unew, or the join pass, has just made unew equal uold.
+                rh::FusedCourant fc;
+                fc.pre_from_uold = c->variant != 0;
                 Timed tm(c, TK_GODUNOV);
                 hipError_t e = rh::launch_godunov(m, lv, gp, c->p.riemann, c->p.precision, c->variant, dts, 0.0,
-                                                  all_exact ? 1.0 : 0.0, c->stream);
+                                                  all_exact ? 1.0 : 0.0, c->stream, fc);
                 if (e == hipErrorNotSupported)
                     return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
                 if (e != hipSuccess) return hip_fail(c, e, "godunov launch");

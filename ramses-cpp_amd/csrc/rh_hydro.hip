@@ -888,6 +888,25 @@ RH_DEV int xcd_block(int x, int n) {
 // variant 2); rh_autotune times both.
 template <int OCTS> constexpr int oct_block() { return OCTS * 8; }
 
+// set_unew (uold -> unew on every cell of the level's octs) with the memory
+// access pattern of k_godunov_oct's own-cell loads and unew stores: the
+// eight cells of an oct in eight adjacent lanes, 8 octs per 64-lane
+// workgroup, workgroups in xcd_block order, so a wave's access to one
+// variable touches eight 64-byte pieces (one per cell index; eight
+// consecutive grids each on a uniform level).  A copy of known size whose
+// byte counters calibrate those of the Godunov kernel.
+__global__ __launch_bounds__(64) void k_set_unew_oct_lanes(MeshDev m, LevelDev lv) {
+    const int t = threadIdx.x;
+    const int p = xcd_block(blockIdx.x, gridDim.x) * 8 + (t >> 3);
+    if (p >= lv.noct) return;
+    const size_t c = (size_t)cell_id(m, lv.grid[p], t & 7) - 1;
+    double v[5];
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) v[iv] = m.uold[(size_t)iv * m.ncell + c];
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) m.unew[(size_t)iv * m.ncell + c] = v[iv];
+}
+
 // Occupancy: the FP64 form runs two waves per SIMD (at most 256 VGPRs), the
 // mixed form three (at most 168).  LDS per oct: the incoming contributions
 // and their keys, 2.25 KB in the FP64 form, and the FP32 incoming fluxes,
@@ -2074,6 +2093,12 @@ hipError_t launch_set_unew(const MeshDev& m, const LevelDev& lv, hipStream_t str
     return hipGetLastError();
 }
 
+hipError_t launch_set_unew_oct_lanes(const MeshDev& m, const LevelDev& lv, hipStream_t stream) {
+    if (lv.noct == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_unew_oct_lanes, dim3((lv.noct + 7) / 8), dim3(64), 0, stream, m, lv);
+    return hipGetLastError();
+}
+
 hipError_t launch_set_uold(const MeshDev& m, const LevelDev& lv, hipStream_t stream) {
     if (lv.noct == 0) return hipSuccess;
     hipLaunchKernelGGL(k_set_uold, dim3(stream_blocks(lv.noct)), dim3(SBLOCK), 0, stream, m, lv);
@@ -2204,11 +2229,12 @@ hipError_t launch_godunov(const MeshDev& m, const LevelDev& lv, const GasParams&
     if (lv.noct == 0) return hipSuccess;
     if (precision != 0 && precision != 1) return hipErrorNotSupported;
     if (variant < 0 || variant >= KERNEL_VARIANTS) return hipErrorInvalidValue;
-    // The fused step needs the oct-in-lanes kernel, a single mesh, and a level
-    // with no coarser neighbours (no reflux targets, no records).
-    if ((fc.part || fc.keys || fc.pre_from_uold) &&
-        (variant == 0 || lv.ntgt != 0 || lv.nrec != 0 || lv.nleaf != 8LL * lv.noct))
+    // The fused Courant epilogue needs the oct-in-lanes or brick kernel and a
+    // level with no coarser neighbours (no reflux targets, no records); the
+    // pre-update state from uold needs one of those kernels.
+    if ((fc.part || fc.keys) && (variant == 0 || lv.ntgt != 0 || lv.nrec != 0 || lv.nleaf != 8LL * lv.noct))
         return hipErrorInvalidValue;
+    if (fc.pre_from_uold && variant == 0) return hipErrorInvalidValue;
     if ((fc.part && (lv.obox || fc.keys)) || (fc.keys && !lv.obox)) return hipErrorInvalidValue;
     const bool mixed = precision == 1;
     const bool exact = inv_dx != 0.0;

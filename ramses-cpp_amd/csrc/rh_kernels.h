@@ -52,6 +52,9 @@ struct GasParams {
 // Launchers (defined in rh_hydro.hip).  All enqueue on `stream`.
 hipError_t launch_set_unew(const MeshDev& m, const LevelDev& lv, hipStream_t stream);
 hipError_t launch_set_uold(const MeshDev& m, const LevelDev& lv, hipStream_t stream);
+// The same copy as launch_set_unew with the Godunov kernel's lane mapping
+// (k_set_unew_oct_lanes), for calibrating byte counters.
+hipError_t launch_set_unew_oct_lanes(const MeshDev& m, const LevelDev& lv, hipStream_t stream);
 // Gravity source terms of the level's leaf cells: the kick into unew
 // (add_gravity_source_terms) and the kick of uold at constant internal energy
 // (synchro_hydro_fine).  m.f null means f = 0.
@@ -98,13 +101,14 @@ hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, dou
 // identical results), else 0; an ensemble reads dx and 1/dx per box and
 // passes inv_dx != 0 when every box has an exact reciprocal.
 constexpr int KERNEL_VARIANTS = 6;
-// Fused step of a single-level run or an ensemble: with pre_from_uold the
-// launch reads the cells' pre-update unew from uold (what set_unew copies
-// into unew), and each leaf cell also takes the Courant candidate of its new
-// state (what set_uold copies into uold) into a block minimum in
+// pre_from_uold: the launch reads the cells' pre-update unew from uold, for
+// a caller that knows unew equals uold on the level's leaf cells (what
+// set_unew copies; variants 1 to 5).  Fused step of a single-level run or an
+// ensemble: each leaf cell also takes the Courant candidate of its new state
+// (what set_uold copies into uold) into a block minimum in
 // part[godunov_blocks(lv, variant)] (single mesh, part non-null) or into
-// its box's key in keys (ensemble, keys non-null; as launch_courant_boxes).
-// Only for variants 1 to 5 on levels with only leaf cells and same-level
+// its box's key in keys (ensemble, keys non-null; as launch_courant_boxes),
+// only for variants 1 to 5 on levels with only leaf cells and same-level
 // neighbours (no records, no reflux targets).
 struct FusedCourant {
     bool pre_from_uold = false;

@@ -147,6 +147,7 @@ def lib() -> C.CDLL:
         L.rh_get_kernel_variant.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.rh_autotune.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int), _DP]
         L.rh_set_graphs.argtypes = [C.c_void_p, C.c_int]
+        L.rh_set_unew_oct_lanes.argtypes = [C.c_void_p, C.c_int]
         L.rh_graph_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.rh_add_gravity_source_terms.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.rh_synchro_hydro_fine.argtypes = [C.c_void_p, C.c_int, C.c_double]
@@ -180,7 +181,7 @@ EXPORTED = [
     "rh_bind_ensemble", "rh_run_ensemble", "rh_ensemble_leaf_cells", "rh_add_gravity_source_terms",
     "rh_synchro_hydro_fine", "rh_amr_create", "rh_amr_destroy", "rh_amr_step", "rh_amr_mesh", "rh_amr_stats",
     "rh_amr_context", "rh_amr_timers", "rh_dump_snapshot", "rh_write_hydro", "rh_write_amr", "rh_amr_dump",
-    "rh_set_graphs", "rh_graph_stats",
+    "rh_set_graphs", "rh_graph_stats", "rh_set_unew_oct_lanes",
 ]
 
 
@@ -568,6 +569,11 @@ class Solver:
         v = C.c_int()
         self._chk(lib().rh_get_kernel_variant(self._ctx, C.byref(v)))
         return v.value
+
+    def set_unew_oct_lanes(self, ilevel):
+        """set_unew with the Godunov kernel's lane mapping (counter
+        calibration); the same result as set_unew."""
+        self._chk(lib().rh_set_unew_oct_lanes(self._ctx, ilevel))
 
     def graphs(self, enable=True):
         """Replay multi-step exchanged-buffer runs from captured HIP graphs

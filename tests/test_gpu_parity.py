@@ -283,3 +283,31 @@ def test_run_uniform_exchanged_buffers_match_copies(precision):
             assert_same(out[v][0], out[0][0], f"dt, {n} steps, variant {v}")
             assert_same(out[v][1], out[0][1], f"uold, {n} steps, variant {v}")
             assert_same(out[v][2], out[0][2], f"unew, {n} steps, variant {v}")
+
+
+def test_set_unew_oct_lanes_equals_set_unew():
+    """The counter-calibration copy (the Godunov kernel's lane mapping) leaves
+    unew exactly as set_unew does, on every level of an adaptive mesh (lists
+    of 1 to some thousands of octs, not a multiple of eight) and on a uniform
+    level."""
+    from amr_meshes import refined_mesh
+
+    meshes = [(refined_mesh(4, spheres=((0.4, 0.5, 0.6, 0.3), (0.45, 0.5, 0.55, 0.15))), range(1, 7)),
+              (_random_mesh(5, 11), [5])]
+    for m0, levels in meshes:
+        for lev in levels:
+            print(f"case set_unew_oct_lanes level {lev}", flush=True)
+            out = []
+            for lanes in (False, True):
+                m = clone(m0)
+                m.unew[:] = -7.0
+                s = rhydro.Solver(gamma=1.4)
+                s.bind(m, rhydro.RH_HOST_COHERENT)
+                if lanes:
+                    s.set_unew_oct_lanes(lev)
+                else:
+                    s.set_unew(lev)
+                s.close()
+                out.append(m.unew.copy())
+            assert_same(out[1], out[0], f"level {lev} unew")
+            assert (out[1] != -7.0).any() or len(m0.level_octs(lev)) == 0
