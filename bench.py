@@ -14,6 +14,7 @@ bounded sample of the same state on this host).
 """
 import argparse
 import json
+import math
 import os
 import platform
 import subprocess
@@ -90,6 +91,17 @@ PROFILE_NOTE = "profiles/godunov_level9_256cubed_hllc_minmod.md (rocprofv3 count
 REFERENCE_FP64_OPS_PER_CELL = {("hllc", 1): 2412.6, ("hllc", 2): 2421.8, ("hll", 1): 2502.7, ("hll", 2): 2512.6,
                                ("llf", 1): 2412.7, ("llf", 2): 2422.6}
 FP64_PEAK_TFLOPS = 78.6
+
+
+def finite_or_null(x):
+    """x with every non-finite float replaced by None (strict JSON)."""
+    if isinstance(x, dict):
+        return {k: finite_or_null(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [finite_or_null(v) for v in x]
+    if isinstance(x, float) and not math.isfinite(x):
+        return None
+    return x
 
 
 def cpu_model() -> str:
@@ -223,12 +235,18 @@ def measure_ensemble(args, rhydro, np):
         for name in ("rho", "p", "v"):
             for norm in ("L1", "Linf"):
                 vals = np.array([per_box[b][name][norm] for b in members])
-                res[f"{name}_{norm}"] = {"max": float(vals.max()), "median": float(np.median(vals)),
-                                         "worst_box": int(members[int(vals.argmax())])}
+                ok = np.isfinite(vals)
+                if not ok.any():
+                    res[f"{name}_{norm}"] = None
+                    continue
+                worst = int(np.flatnonzero(ok)[int(vals[ok].argmax())])
+                res[f"{name}_{norm}"] = {"max": float(vals[ok].max()), "median": float(np.median(vals[ok])),
+                                         "worst_box": int(members[worst]), "nonfinite_boxes": int((~ok).sum())}
         return res
 
-    err_summary = {g: summary([b for b in range(len(boxes)) if group[b] == g])
-                   for g in ("sod", "sedov", "sedov_collapsed")}
+    err_summary = {g: summary([b for b in range(len(boxes)) if group[b] == g]) for g in ("sod", "sedov")}
+    blown = [b for b in range(len(boxes)) if group[b] == "sedov_collapsed"]
+    err_summary["sedov_blown_up"] = {"boxes": len(blown), "box_ids": blown}
     return {
         "workload": f"box ensemble: {args.boxes} periodic cubic boxes, sides {sides[0]}-{sides[-1]} cells "
                     f"(log-uniform, nx*2^(L-1) with nx in 1,3,5,7; seed {args.seed}), {nleaf} leaf cells; "
@@ -248,8 +266,8 @@ def measure_ensemble(args, rhydro, np):
         "mixed_error_vs_fp64_per_box": err_summary,
         "error_note": "relative errors of the mixed form against the FP64 form on each box's leaf cells after "
                       "warmup + steps, each form with its own time steps; max and median over the shock tubes, "
-                      "the Sedov boxes, and apart the Sedov boxes whose FP64 run (the reference scheme's) has "
-                      "blown up (a time step below 1e-6 of its first)",
+                      "the Sedov boxes; the Sedov boxes whose FP64 run (the reference scheme's) has blown up "
+                      "(a time step below 1e-6 of its first) are only listed",
     }
 
 
@@ -495,7 +513,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--level", type=int, default=9, help="reference level list (9 -> 256^3 cells)")
     ap.add_argument("--riemann", default="hllc")
     ap.add_argument("--slope", type=int, default=1)
@@ -530,7 +548,7 @@ def main() -> int:
         else:
             rec = measure_amr(args, rhydro, np, "sedov" if args.only == "amr_sedov" else "kh")
             rec.pop("_state", None)
-        print(json.dumps({"only": args.only, "record": rec}))
+        print(json.dumps(finite_or_null({"only": args.only, "record": rec}), allow_nan=False))
         return 0
 
     gamma = 5.0 / 3.0
@@ -753,7 +771,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    print(json.dumps(finite_or_null(out), allow_nan=False))
     return 0
 
 

@@ -1,5 +1,7 @@
-"""The mixed-precision form (FP32 slopes / trace / wave structure, FP64
-prim->cons, flux assembly, reflux and update) against the FP64 oracle.
+"""The mixed-precision form (FP32 face states -- the primitive conversion
+of each cell on a line, slopes, trace -- and an FP32 Riemann solver and face
+fluxes; FP64 flux difference, reflux, update, clamps and Courant step)
+against the FP64 oracle.
 
 The mixed form is not bit-faithful by design.  On smooth states one step
 from identical input must agree with the FP64 oracle to FP32 accuracy of the
@@ -61,6 +63,25 @@ def test_mixed_single_step_smooth(riemann, slope):
     # dt/dx ~ 0.1: per-step errors stay below 1e-6 relative.
     for k, (l1, linf) in err.items():
         assert l1 < 1e-7 and linf < 1e-5, (k, l1, linf)
+
+
+@pytest.mark.parametrize("riemann", ["hll", "hllc"])
+def test_mixed_ten_steps_smooth(riemann):
+    """Ten steps on a smooth state, each form with its own time steps: the
+    per-step FP32 flux errors accumulate to at most L1 1e-6 and Linf 1e-4
+    relative in density and pressure."""
+    level = 5
+    m = rhydro.uniform_mesh(level)
+    rhydro.init_lognormal(m, level, 321, 5.0 / 3.0, 0.1, 0.1)
+    print(f"case mixed ten steps {riemann}", flush=True)
+    mg, mo, dtg, dto = _run(m, level, riemann, 1, 10, gamma=5.0 / 3.0)
+    assert np.isfinite(mg.uold).all()
+    np.testing.assert_allclose(dtg, dto, rtol=1e-5)
+    err = _errors(mg, mo, level, 5.0 / 3.0)
+    print("mixed ten-step errors", err, flush=True)
+    for k in ("rho", "p"):
+        l1, linf = err[k]
+        assert l1 < 1e-6 and linf < 1e-4, (k, l1, linf)
 
 
 def test_mixed_state_at_rest_unchanged():
