@@ -552,29 +552,35 @@ void ensure_ref_rules(rh_amr* a, const std::vector<int>& grids) {
         if (k == 0 || father_oct(grids[k]) != father_oct(grids[k - 1]) || father_oct(grids[k]) == 0)
             start.push_back((int)k);
     start.push_back((int)grids.size());
-    parallel_for((int)start.size() - 1, [&](int r) {
-        int block[64];
-        bool have = false;
-        for (int k = start[r]; k < start[r + 1]; ++k) {
-            const int ig = grids[k];
-            const int fc = a->father[ig - 1];
-            bool ok;
-            if (fc <= 0) {
-                ok = false;
-            } else if (fc <= a->ncoarse || !cubic) {
-                ok = neighbours_refined_by_search(a, fc);
-            } else {
-                const int g = ((fc - a->ncoarse - 1) % a->ngridmax) + 1;
-                if (!have) {
-                    cells_around(a, g, block);
-                    have = true;
+    // A run costs 64 neighbour walks and, next to a coarser region, up to
+    // eight coordinate searches (microseconds), so the worker threads take
+    // lists of a few dozen runs already.
+    parallel_for(
+        (int)start.size() - 1,
+        [&](int r) {
+            int block[64];
+            bool have = false;
+            for (int k = start[r]; k < start[r + 1]; ++k) {
+                const int ig = grids[k];
+                const int fc = a->father[ig - 1];
+                bool ok;
+                if (fc <= 0) {
+                    ok = false;
+                } else if (fc <= a->ncoarse || !cubic) {
+                    ok = neighbours_refined_by_search(a, fc);
+                } else {
+                    const int g = ((fc - a->ncoarse - 1) % a->ngridmax) + 1;
+                    if (!have) {
+                        cells_around(a, g, block);
+                        have = true;
+                    }
+                    ok = neighbours_refined_in_block(a, g, (fc - a->ncoarse - 1) / a->ngridmax, block);
                 }
-                ok = neighbours_refined_in_block(a, g, (fc - a->ncoarse - 1) / a->ngridmax, block);
+                if (!ok)
+                    for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
             }
-            if (!ok)
-                for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
-        }
-    });
+        },
+        32);
 }
 
 // Gradient criteria of the coarse cells (TreeUpdater.cpp:459-480), on the

@@ -87,20 +87,35 @@ void nbor_cells_of_cell(const rh_mesh& m, int icell, int icn[6]) {
 
 int build_topo_index(const rh_mesh& m, TopoIndex& ti, std::string& err) {
     MeshView v{m};
-    // Every index the device kernels will follow must stay inside its array.
-    for (int i = 0; i < m.ncell; ++i)
-        if (m.son[i] < 0 || m.son[i] > m.ngridmax) {
-            err = "son[" + std::to_string(i + 1) + "] = " + std::to_string(m.son[i]) + " outside 0..ngridmax";
-            return RH_ERR_TOPOLOGY;
+    // Every index the table builder and the kernels follow from the level
+    // lists must stay inside its array: the coarse cells' sons, and for every
+    // grid in a list its father, its list link and its cells' sons.  (Grids
+    // in no list are never reached from one.)
+    auto son_ok = [&](int icell) {
+        const int s = m.son[icell - 1];
+        if (s >= 0 && s <= m.ngridmax) return true;
+        err = "son[" + std::to_string(icell) + "] = " + std::to_string(s) + " outside 0..ngridmax";
+        return false;
+    };
+    for (int i = 1; i <= m.ncoarse; ++i)
+        if (!son_ok(i)) return RH_ERR_TOPOLOGY;
+    // The index arrays keep their size between builds for one ngridmax, and
+    // only the entries the previous build wrote are cleared.
+    if (ti.glev.size() != (size_t)m.ngridmax + 1) {
+        ti.glev.assign((size_t)m.ngridmax + 1, 0);
+        ti.gpos.assign((size_t)m.ngridmax + 1, -1);
+        ti.depth.assign((size_t)m.ngridmax + 1, 0);
+    } else {
+        for (const int g : ti.touched) {
+            ti.glev[g] = 0;
+            ti.gpos[g] = -1;
+            ti.depth[g] = 0;
         }
-    for (int g = 0; g < m.ngridmax; ++g)
-        if (m.father[g] < 0 || m.father[g] > m.ncell || m.next[g] < 0 || m.next[g] > m.ngridmax) {
-            err = "father/next of grid " + std::to_string(g + 1) + " out of range";
-            return RH_ERR_TOPOLOGY;
-        }
-    ti.glev.assign((size_t)m.ngridmax + 1, 0);
-    ti.gpos.assign((size_t)m.ngridmax + 1, -1);
-    ti.depth.assign((size_t)m.ngridmax + 1, 0);
+    }
+    ti.touched.clear();
+    ti.epoch = -1;
+    std::vector<int>& listed = ti.listed;
+    listed.clear();
     for (int il = 1; il <= m.nlevelmax; ++il) {
         int ig = v.headl(il);
         int pos = 0;
@@ -113,8 +128,17 @@ int build_topo_index(const rh_mesh& m, TopoIndex& ti, std::string& err) {
                 err = "grid " + std::to_string(ig) + " appears in two level lists (or a list is cyclic)";
                 return RH_ERR_TOPOLOGY;
             }
+            if (m.father[ig - 1] < 0 || m.father[ig - 1] > m.ncell || m.next[ig - 1] < 0 ||
+                m.next[ig - 1] > m.ngridmax) {
+                err = "father/next of grid " + std::to_string(ig) + " out of range";
+                return RH_ERR_TOPOLOGY;
+            }
+            for (int ic = 0; ic < 8; ++ic)
+                if (!son_ok(m.ncoarse + ic * m.ngridmax + ig)) return RH_ERR_TOPOLOGY;
             ti.glev[ig] = il;
             ti.gpos[ig] = pos++;
+            listed.push_back(ig);
+            ti.touched.push_back(ig);
             ig = m.next[ig - 1];
         }
     }
@@ -124,7 +148,7 @@ int build_topo_index(const rh_mesh& m, TopoIndex& ti, std::string& err) {
     // already known, so the whole pass is linear in the grids.  Grids with
     // father 0 keep depth 0 (the reference never asks for theirs).
     std::vector<int> chain;
-    for (int ig = 1; ig <= m.ngridmax; ++ig) {
+    for (const int ig : listed) {
         if (m.father[ig - 1] <= 0 || ti.depth[ig] != 0) continue;
         chain.clear();
         int cur = ig, base = 0;
@@ -147,7 +171,10 @@ int build_topo_index(const rh_mesh& m, TopoIndex& ti, std::string& err) {
             return RH_ERR_TOPOLOGY;
         }
         for (int k = (int)chain.size() - 1, d = base + 1; k >= 0; --k, ++d)
-            if (m.father[chain[k] - 1] > 0) ti.depth[chain[k]] = d;
+            if (m.father[chain[k] - 1] > 0) {
+                ti.depth[chain[k]] = d;
+                if (ti.glev[chain[k]] == 0) ti.touched.push_back(chain[k]);
+            }
     }
     ti.epoch = m.mesh_epoch;
     return RH_OK;

@@ -35,6 +35,8 @@ struct TopoIndex {
     std::vector<int> glev;   // list index of each grid (0 = in no list)
     std::vector<int> gpos;   // position of each grid within its list
     std::vector<int> depth;  // father-chain depth = the reference's cell_levels value
+    std::vector<int> listed; // every grid of the level lists, list by list
+    std::vector<int> touched;  // grids whose entries above the build wrote
 };
 
 struct LevelHost {
