@@ -143,3 +143,24 @@ def test_kelvin_helmholtz_amr_run_scaled():
     assert g.state()["numbl"][9] > 0
     g.close()
     o.close()
+
+
+def test_kelvin_helmholtz_initial_mesh_at_benchmark_levels():
+    """The benchmark's adaptive Kelvin-Helmholtz slab at its own levels
+    (128^3 base at reference level 8, refined to level 11 on density
+    gradients, err_grad_d 0.2; 2,097,152 octs on level 11): the initial
+    refinement -- every level list in order, the links, the geometry and the
+    FP64 state, restricted upward -- identical to the oracle's restatement.
+    (A coarse step at this size takes the oracle minutes; the same step is
+    compared two levels coarser above.)"""
+    kw = dict(levelmin=8, levelmax=11, ngridmax=4500000, err_grad_d=0.2, ic="kh", kh_amp=0.01, riemann="hllc")
+    print("case amr run kh levels 8-11, initial mesh", flush=True)
+    g = rhydro.AmrRun(**kw)
+    gs = g.state()
+    g.close()
+    print("gpu run initialised", flush=True)
+    o = oracle.OracleAmr(**{k: (rhydro.resolve_riemann(v) if k == "riemann" else v) for k, v in kw.items()})
+    print("oracle initialised", flush=True)
+    _compare(gs, o.state(), "initial mesh")
+    assert gs["numbl"][11] == 2097152
+    o.close()
