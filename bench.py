@@ -204,7 +204,7 @@ def measure_ensemble(args, rhydro, np):
         s.run_uniform(levels[b], dxs[b], cf, args.steps)
         t_loop += time.perf_counter() - t0
         s.sync_to_host()
-        same = same and bool(np.array_equal(m.uold, finals["mixed"][b]))
+        same = same and bool(np.array_equal(m.uold, finals["mixed"][b], equal_nan=True))
     s.close()
     unbatched = nleaf * args.steps / t_loop
     print(f"[bench] unbatched mixed loop: {t_loop:.4f}s, identical={same}", file=sys.stderr, flush=True)
@@ -355,7 +355,8 @@ def measure_amr(args, rhydro, np, which, precision=None):
         tcpu = time.perf_counter() - tcpu0
         g.step(ncmp)
         so, sg = o.state(), g.state()
-        same = all(np.array_equal(so[k], sg[k]) for k in ("numbl", "son", "father", "nbor", "next", "uold", "unew"))
+        same = all(np.array_equal(so[k], sg[k], equal_nan=True)
+                   for k in ("numbl", "son", "father", "nbor", "next", "uold", "unew"))
         out.update({"cpu_value": so["cell_updates"] / tcpu, "cpu_kind": "port", "cpu_cores": 1,
                     "cpu_sample": f"the first {ncmp} coarse steps through the oracle",
                     "identical_to_oracle_after_2_steps": bool(same)})
@@ -442,7 +443,8 @@ def measure_sedov64(args, rhydro, np):
         dto = om.run_uniform(oracle.params(1.4, 1, 2), level, dx, cf, nsteps)
         tcpu = time.perf_counter() - tc
         out.update({"cpu_value": nleaf * nsteps / tcpu, "cpu_wall_s": tcpu, "cpu_kind": "port", "cpu_cores": 1,
-                    "identical_to_oracle": bool(np.array_equal(dts, dto) and np.array_equal(mesh.uold, ref.uold))})
+                    "identical_to_oracle": bool(np.array_equal(dts, dto, equal_nan=True)
+                                                and np.array_equal(mesh.uold, ref.uold, equal_nan=True))})
     print(f"[bench] sedov64: gpu {wall:.4f}s" + (f", cpu {out['cpu_wall_s']:.1f}s, identical "
                                                  f"{out['identical_to_oracle']}" if "cpu_wall_s" in out else ""),
           file=sys.stderr, flush=True)
@@ -505,7 +507,7 @@ def measure_dropin(args, rhydro, np, which):
                           "topology_check_ms_per_step": (wall - calls) / nsteps * 1e3}
         print(f"[bench] drop-in {which} {mode_name}: {wall / nsteps * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
     out["cells"] = int(nleaf)
-    out["modes_identical"] = bool(np.array_equal(finals["host_coherent"], finals["device_resident"]))
+    out["modes_identical"] = bool(np.array_equal(finals["host_coherent"], finals["device_resident"], equal_nan=True))
     return out
 
 

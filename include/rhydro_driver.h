@@ -76,12 +76,18 @@ int rh_timers_reset(rh_ctx* ctx);
 /* Godunov kernel variant of a context: 0 = one thread per cell with the
  * octs of a workgroup strided across lanes; 1 = the eight cells of an oct in
  * adjacent lanes, sharing sibling states and fluxes by cross-lane shuffles,
- * 16 octs per workgroup; 2 (default) = the same with 8 octs per workgroup;
+ * 16 octs per workgroup; 2 = the same with 8 octs per workgroup;
  * 3, 4, 5 = bricks of 16, 32, 64 consecutive octs per workgroup that also
  * share face states and fluxes between neighbouring octs through LDS (a
- * 2x2x4, 4x4x2, 4x4x4 brick of octs on a uniform level).  All produce
- * identical results. */
-#define RH_KERNEL_VARIANTS 6
+ * 2x2x4, 4x4x2, 4x4x4 brick of octs on a uniform level); 6 = 8 octs per
+ * workgroup of three waves, each computing one dimension of the update for
+ * the same cells.  All produce identical results.  -1 (the default of a new
+ * context) chooses per level: 6 for the FP64 form, and for the mixed form 6
+ * on levels of fewer than 16,384 octs and 2 on larger ones (the fastest
+ * measured at 256^3 cells and on small levels).  rh_autotune and
+ * rh_set_kernel_variant fix one variant for every level of the context;
+ * rh_get_kernel_variant reports -1 while the default is in use. */
+#define RH_KERNEL_VARIANTS 7
 int rh_set_kernel_variant(rh_ctx* ctx, int variant);
 int rh_get_kernel_variant(rh_ctx* ctx, int* variant);
 

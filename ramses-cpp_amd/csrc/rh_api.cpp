@@ -286,6 +286,11 @@ double exact_inverse(double dx) {
     return 1.0 / dx;
 }
 
+int variant_for(const rh_ctx* c, const rh::LevelDev& lv) {
+    if (c->variant >= 0) return c->variant;
+    return (c->p.precision == RH_PREC_FP64 || lv.noct < AUTO_DIMS_MAX_OCTS) ? 6 : 2;
+}
+
 void drop_graph(StepGraph& g) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     g.exec = nullptr;
@@ -341,9 +346,10 @@ int do_courant(rh_ctx* c, const rh::LevelDev* lv, double dx, double gamma, doubl
 int do_godunov(rh_ctx* c, const rh::LevelDev* lv, const double* d_dt, double dx, bool pre_from_uold = false) {
     Timed tm(c, TK_GODUNOV);
     rh::FusedCourant fc;
-    fc.pre_from_uold = pre_from_uold && c->variant != 0;
-    hipError_t e = rh::launch_godunov(mesh_dev(c), *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision,
-                                      c->variant, d_dt, dx, exact_inverse(dx), c->stream, fc);
+    const int var = variant_for(c, *lv);
+    fc.pre_from_uold = pre_from_uold && var != 0;
+    hipError_t e = rh::launch_godunov(mesh_dev(c), *lv, gas(c, c->p.gamma), c->p.riemann, c->p.precision, var,
+                                      d_dt, dx, exact_inverse(dx), c->stream, fc);
     if (e == hipErrorNotSupported) return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
     if (e != hipSuccess) return hip_fail(c, e, "godunov launch");
     tm.done();
@@ -697,10 +703,11 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
         // reads, set_uold's copy is the exchange) and computes the next step's
         // Courant candidates from the values it writes.  At the end the level's
         // cells of both buffers hold the final state, as after set_uold.
-        const bool fused = c->variant != 0 && lv->ntgt == 0 && lv->nrec == 0 && lv->nleaf == 8LL * lv->noct &&
+        const int var = variant_for(c, *lv);
+        const bool fused = var != 0 && lv->ntgt == 0 && lv->nrec == 0 && lv->nleaf == 8LL * lv->noct &&
                            lv->noct > 0;
         if (fused && nsteps > 0) {
-            const int nb = rh::godunov_blocks(*lv, c->variant);
+            const int nb = rh::godunov_blocks(*lv, var);
             // Both sizes up front: the launches may be captured into a graph,
             // where no allocation may happen.
             if ((st = ensure_part(c, std::max(nb, rh::courant_blocks(*lv)) + 1))) return st;
@@ -718,7 +725,7 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
                     fc.courant_factor = courant_factor;
                     {
                         Timed tm(c, TK_GODUNOV);
-                        hipError_t e = rh::launch_godunov(mk, *lv, gp, c->p.riemann, c->p.precision, c->variant,
+                        hipError_t e = rh::launch_godunov(mk, *lv, gp, c->p.riemann, c->p.precision, var,
                                                           c->d_dt + k, dx, inv_dx, c->stream, fc);
                         if (e == hipErrorNotSupported)
                             return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
@@ -743,7 +750,7 @@ int rh_run_uniform(rh_ctx* c, int ilevel, double dx, double courant_factor, int 
             key_add(key, *lv);
             key_add(key, m);
             key_add(key, gp);
-            key.add(c->p.riemann).add(c->p.precision).add(c->variant).add(dx).add(inv_dx).add(courant_factor);
+            key.add(c->p.riemann).add(c->p.precision).add(var).add(dx).add(inv_dx).add(courant_factor);
             key.add(dt_init).add(nsteps).add(nb).add(c->d_dt).add(c->d_part).add(c->npart);
             if ((st = run_graph(c, c->g_uniform, key, issue))) return st;
         }
@@ -807,7 +814,7 @@ int rh_graph_stats(rh_ctx* c, long long* captured, long long* launched) {
 int rh_set_kernel_variant(rh_ctx* c, int variant) {
     return guard(c ? &c->err : &g_create_error, nullptr, [&]() -> int {
         if (!c) return RH_ERR_ARG;
-        if (variant < 0 || variant >= RH_KERNEL_VARIANTS) return fail(c, RH_ERR_ARG, "unknown kernel variant");
+        if (variant < -1 || variant >= RH_KERNEL_VARIANTS) return fail(c, RH_ERR_ARG, "unknown kernel variant");
         c->variant = variant;
         return RH_OK;
     });

@@ -87,7 +87,9 @@ struct rh_ctx {
     double* d_u1 = nullptr;
     double* d_u2 = nullptr;
     int interpol_cap = 0;
-    int variant = 2;  // Godunov kernel variant (rh_set_kernel_variant)
+    // Godunov kernel variant (rh_set_kernel_variant, rh_autotune); -1: the
+    // default, chosen per level by variant_for.
+    int variant = -1;
     // Device address of the allocation-free Courant step's scratch array on
     // this context's GPU (looked up by rh_create), or null with the lookup's
     // error in scratch_err.
@@ -156,6 +158,13 @@ int godunov_dev(rh_ctx* c, int ilevel, const double* d_dt, double dx, bool unew_
 int set_uold_restrict_dev(rh_ctx* c, int ilevel, const int* d_father);
 // Frees every device buffer of the current binding (single or ensemble).
 void unbind(rh_ctx* c);
+
+// The Godunov kernel variant of a launch on lv: the context's when one is
+// set, else the measured default -- one wave per dimension (variant 6) for
+// the FP64 form, and for the mixed form on levels of fewer than
+// AUTO_DIMS_MAX_OCTS octs; 8 octs per workgroup (variant 2) otherwise.
+constexpr int AUTO_DIMS_MAX_OCTS = 16384;
+int variant_for(const rh_ctx* c, const rh::LevelDev& lv);
 
 void drop_graph(StepGraph& g);
 // The launch-relevant values of a level's tables, a mesh and the gas

@@ -298,7 +298,8 @@ int rh_run_ensemble(rh_ctx* c, const int* levels, const double* dx, double coura
         // launch taking the next step's per-box Courant keys from the values it
         // writes; a final copy leaves the last state in both buffers.  Those
         // launches are replayed from a captured graph (run_graph).
-        const bool exchanged = c->variant != 0 && lv.ntgt == 0 && lv.nrec == 0 && lv.nleaf == 8LL * lv.noct &&
+        const int var = variant_for(c, lv);
+        const bool exchanged = var != 0 && lv.ntgt == 0 && lv.nrec == 0 && lv.nleaf == 8LL * lv.noct &&
                                lv.noct > 0;
         if (exchanged && nsteps > 0) {
             auto issue = [&]() -> int {
@@ -320,7 +321,7 @@ int rh_run_ensemble(rh_ctx* c, const int* levels, const double* dx, double coura
                     fc.courant_factor = courant_factor;
                     {
                         Timed tm(c, TK_GODUNOV);
-                        hipError_t e = rh::launch_godunov(mk, lv, gp, c->p.riemann, c->p.precision, c->variant, dts,
+                        hipError_t e = rh::launch_godunov(mk, lv, gp, c->p.riemann, c->p.precision, var, dts,
                                                           0.0, all_exact ? 1.0 : 0.0, c->stream, fc);
                         if (e == hipErrorNotSupported)
                             return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");
@@ -343,7 +344,7 @@ int rh_run_ensemble(rh_ctx* c, const int* levels, const double* dx, double coura
             key_add(key, lv);
             key_add(key, m);
             key_add(key, gp);
-            key.add(c->p.riemann).add(c->p.precision).add(c->variant).add(all_exact).add(courant_factor);
+            key.add(c->p.riemann).add(c->p.precision).add(var).add(all_exact).add(courant_factor);
             key.add(dt_init).add(nsteps).add(nbox).add(nk).add(c->d_keys).add(c->d_dt);
             if ((st = run_graph(c, c->g_ensemble, key, issue))) return st;
         } else {
@@ -368,9 +369,9 @@ int rh_run_ensemble(rh_ctx* c, const int* levels, const double* dx, double coura
             {
                 // set_unew, or the join pass, has just made unew equal uold.
                 rh::FusedCourant fc;
-                fc.pre_from_uold = c->variant != 0;
+                fc.pre_from_uold = var != 0;
                 Timed tm(c, TK_GODUNOV);
-                hipError_t e = rh::launch_godunov(m, lv, gp, c->p.riemann, c->p.precision, c->variant, dts, 0.0,
+                hipError_t e = rh::launch_godunov(m, lv, gp, c->p.riemann, c->p.precision, var, dts, 0.0,
                                                   all_exact ? 1.0 : 0.0, c->stream, fc);
                 if (e == hipErrorNotSupported)
                     return fail(c, RH_ERR_UNSUPPORTED, "riemann/precision combination not supported");

@@ -1204,6 +1204,262 @@ void k_godunov_oct(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt,
 }
 
 // ---------------------------------------------------------------------------
+// Dimension-parallel variant (variant 6), for levels with few octs, whose
+// update is bound by the length of one wave's instruction stream rather than
+// by throughput: three waves per 8 octs, wave d computing dimension d of the
+// oct-in-lanes update for the same 64 cells -- its own fluxes on the two
+// faces along d and the two incoming contributions of d, exactly as one
+// iteration of k_godunov_oct's dimension loop -- then, after a barrier, the
+// first wave adds the three dimensions' own fluxes in dimension order and
+// applies the contributions in processing order.  The same operations on
+// the same operands as k_godunov_oct, so the results are identical; each
+// wave executes about a third of the instructions.  `pre`, cpart, ckeys and
+// courant_factor as in k_godunov_oct (cpart: one minimum per workgroup).
+template <int RIEMANN, bool MIXED, bool EXACT>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2, 8)))
+void k_godunov_dims(MeshDev m, LevelDev lv, GasParams gp, const double* d_dt, double dx, double inv_dx,
+                    const double* pre, double* cpart, unsigned long long* ckeys, double courant_factor) {
+    using TC = typename std::conditional<MIXED, float, double>::type;
+    // Contributions and keys as in k_godunov_oct (slots 2d, 2d+1 written by
+    // wave d), and the own fluxes of dimensions 1 and 2 (minus face, then
+    // plus face) for the first wave.
+    __shared__ TC s_f[30][64];
+    __shared__ long long s_k[MIXED ? 1 : 6][64];
+    __shared__ TC s_own[2][10][64];
+    const int t = threadIdx.x & 63;
+    const int d = threadIdx.x >> 6;  // the same for every lane of a wave
+    const int ic0 = t & 7;
+    const int praw = xcd_block(blockIdx.x, gridDim.x) * 8 + (t >> 3);
+    const int noct = lv.noct;
+    // Lanes past the end of the list work on the last oct (valid memory) and
+    // write nothing, so every lane of an oct group takes part in the shuffles.
+    const bool valid = praw < noct;
+    const int p = valid ? praw : noct - 1;
+    const int lm = lv.leaf[p];
+    const bool leaf = valid && ((lm >> ic0) & 1);
+
+    const Gas<double> gd{gp.gamma, gp.gm1, gp.smalle, gp.smallp, 1.0 / gp.gm1};
+    const Gas<TC> gc{(TC)gp.gamma, (TC)gp.gm1, (TC)gp.smalle, (TC)gp.smallp, (TC)(1.0 / gp.gm1)};
+    const int slope_type = gp.slope_type, interpol_type = gp.interpol_type;
+    double dt;
+    step_of(lv, p, d_dt, dt, dx, inv_dx);
+    const double dtdx = dt / dx;
+    const TC dtc = (TC)dt, dxc = (TC)dx;
+    const Len<TC, EXACT> len{dxc, (TC)inv_dx};
+    const int cC = cell_id(m, lv.grid[p], ic0);
+
+    double uC[5];
+    TC qC[5];
+    load_u(m.uold, m.ncell, cC, uC);
+    face_prim(uC, qC, gd, gc);
+
+    auto prim = [&](int icell, const TC fallback[5], TC q[5]) {
+        if (icell > 0) {
+            double u[5];
+            load_u(m.uold, m.ncell, icell, u);
+            face_prim(u, q, gd, gc);
+        } else {
+            copy5(fallback, q);
+        }
+    };
+
+    // Across the oct face along d: the neighbour oct E (or a record), the
+    // cell X beyond the face and, when X is a leaf, the cell X2 beyond it.
+    const int b = 1 << d;
+    const bool par = (ic0 >> d) & 1;  // 0: sibling on the plus side, X on the minus side
+    const int icS = ic0 ^ b;
+    const int E = lv.nbo[(size_t)(2 * d + (par ? 1 : 0)) * noct + p];
+    int4 rc = make_int4(0, 0, 0, 0);
+    int Xcell, X2cell = 0;
+    if (E >= 0) {
+        const int g = lv.grid[E];
+        Xcell = cell_id(m, g, icS);
+        if ((lv.leaf[E] >> icS) & 1) X2cell = cell_id(m, g, ic0);
+    } else {
+        rc = lv.rec[-1 - E];
+        Xcell = rc.x;
+    }
+    double uX[5], uX2[5];
+    if (Xcell > 0) load_u(m.uold, m.ncell, Xcell, uX);
+    if (X2cell > 0) load_u(m.uold, m.ncell, X2cell, uX2);
+
+    TC qS[5], qX[5];
+    shfl5(qC, qS, b);
+    if (Xcell > 0) face_prim(uX, qX, gd, gc);
+    else copy5(qC, qX);
+    TC qL1[5], qR1[5];
+    sel5(par, qS, qX, qL1);
+    sel5(par, qX, qS, qR1);
+    TC OCm[5], OCp[5], GCm[5], GCp[5];
+    own_trace(qL1, qC, qR1, d, slope_type, dtc, len, OCm, OCp, gc);
+    ghost_trace(qL1, qC, qR1, d, interpol_type, dtc, len, GCm, GCp, gc);
+    TC GSm[5], GSp[5];
+    shfl5(GCm, GSm, b);
+    shfl5(GCp, GSp, b);
+
+    // Internal face: this cell's own flux on the sibling side.
+    const bool sib_leaf = (lm >> icS) & 1;
+    TC Fint[5];
+    {
+        TC ql[5], qr[5];
+        sel5(par, GSm, OCm, ql);
+        sel5(par, OCp, GSp, qr);
+        solve_face<RIEMANN, MIXED>(ql, qr, Fint, gc, gd);
+        rotate(Fint, d);
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) Fint[iv] = sib_leaf ? Fint[iv] : TC(0);
+    }
+    TC Fsib[5];
+    shfl5(Fint, Fsib, b);  // the sibling's own flux on this cell's side
+
+    // External face (across the oct boundary).
+    TC Fext[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+    TC Fin[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+    long long Kin = -1;
+    if (E >= 0) {
+        if (X2cell > 0) {
+            TC qX2[5];
+            face_prim(uX2, qX2, gd, gc);
+            TC a5[5], c5[5];
+            sel5(par, qC, qX2, a5);
+            sel5(par, qX2, qC, c5);
+            TC GXm[5], GXp[5], OXm[5], OXp[5];
+            ghost_trace(a5, qX, c5, d, interpol_type, dtc, len, GXm, GXp, gc);
+            own_trace(a5, qX, c5, d, slope_type, dtc, len, OXm, OXp, gc);
+            TC ql[5], qr[5];
+            sel5(par, OCm, GXm, ql);
+            sel5(par, GXp, OCp, qr);
+            solve_face<RIEMANN, MIXED>(ql, qr, Fext, gc, gd);
+            rotate(Fext, d);
+            sel5(par, GCm, OXm, ql);
+            sel5(par, OXp, GCp, qr);
+            solve_face<RIEMANN, MIXED>(ql, qr, Fin, gc, gd);
+            rotate(Fin, d);
+            Kin = ((long long)E * 8 + icS) * 8 + (par ? 2 * d : 2 * d + 1);
+        }
+    } else if (rc.w & REC_REFINED_DEV) {
+    } else if (rc.w & REC_BOUNDARY_DEV) {
+        double u[5];
+        copy5(uC, u);
+#pragma unroll
+        for (int i = 1; i <= 3; ++i)
+            if (i == 1 + d) u[i] = u[i] * -1.0;
+        TC qb[5];
+        face_prim(u, qb, gd, gc);
+        rotate(qb, d);
+        TC dq0[5] = {TC(0), TC(0), TC(0), TC(0), TC(0)};
+        TC Bm[5], Bp[5];
+        trace(qb, dq0, dtc, dxc, Bm, Bp, gc);
+        TC ql[5], qr[5];
+        sel5(par, OCm, Bm, ql);
+        sel5(par, Bp, OCp, qr);
+        solve_face<RIEMANN, MIXED>(ql, qr, Fext, gc, gd);
+        rotate(Fext, d);
+    } else {
+        TC qnl[5], qnr[5];
+        prim(rc.y, qX, qnl);
+        prim(rc.z, qX, qnr);
+        TC Gm[5], Gp[5];
+        ghost_trace(qnl, qX, qnr, d, interpol_type, dtc, len, Gm, Gp, gc);
+        TC ql[5], qr[5];
+        sel5(par, OCm, Gm, ql);
+        sel5(par, Gp, OCp, qr);
+        solve_face<RIEMANN, MIXED>(ql, qr, Fext, gc, gd);
+        rotate(Fext, d);
+        if (leaf) {
+            const int slot = lv.rec_slot[-1 - E] + face_slot(ic0, d);
+            const double sg = par ? -1.0 : 1.0;
+#pragma unroll
+            for (int iv = 0; iv < 5; ++iv) lv.slot_r[(size_t)slot * 5 + iv] = dtdx * sg * Fext[iv] * 0.125;
+        }
+    }
+
+    // Own fluxes in face order (minus face, then plus face).
+    TC Fm[5], Fp[5];
+    sel5(par, Fint, Fext, Fm);
+    sel5(par, Fext, Fint, Fp);
+    if (d > 0) {
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            s_own[d - 1][iv][t] = Fm[iv];
+            s_own[d - 1][5 + iv][t] = Fp[iv];
+        }
+    }
+    // Incoming fluxes: the sibling's own flux on the shared face and X's.
+    const int slotS = 2 * d + (par ? 0 : 1);
+    const int slotX = 2 * d + (par ? 1 : 0);
+    if constexpr (MIXED) {
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            s_f[5 * slotS + iv][t] = Fsib[iv];
+            s_f[5 * slotX + iv][t] = Fin[iv];
+        }
+    } else {
+        const double sgS = par ? -1.0 : 1.0;
+        const double sgX = par ? 1.0 : -1.0;
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            s_f[5 * slotS + iv][t] = dtdx * sgS * Fsib[iv] * 0.125;
+            s_f[5 * slotX + iv][t] = dtdx * sgX * Fin[iv] * 0.125;
+        }
+        s_k[slotS][t] = sib_leaf ? ((long long)p * 8 + icS) * 8 + (par ? 2 * d + 1 : 2 * d) : -1;
+        s_k[slotX][t] = Kin;
+    }
+    __syncthreads();
+    if (d != 0) return;
+
+    // The first wave: the own fluxes of the three dimensions in order, then
+    // the ordered application of the six contributions.
+    double fs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int iv = 0; iv < 5; ++iv) {
+        fs[iv] = fs[iv] + 1.0 * (double)Fm[iv];
+        fs[iv] = fs[iv] + -1.0 * (double)Fp[iv];
+    }
+#pragma unroll
+    for (int dd = 0; dd < 2; ++dd)
+#pragma unroll
+        for (int iv = 0; iv < 5; ++iv) {
+            fs[iv] = fs[iv] + 1.0 * (double)s_own[dd][iv][t];
+            fs[iv] = fs[iv] + -1.0 * (double)s_own[dd][5 + iv][t];
+        }
+    double upre[5];
+    load_u(pre, m.ncell, cC, upre);
+    long long kin[6];
+    if constexpr (MIXED) {
+        incoming_keys(lv, p, ic0, lm, kin);
+    } else {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) kin[f] = s_k[f][t];
+    }
+    if (!cpart && !ckeys) {
+        if (!leaf) return;
+        ordered_apply_flux<64, !MIXED>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, upre, nullptr);
+        return;
+    }
+    double cand = __builtin_inf();
+    if (leaf) {
+        double un[5];
+        ordered_apply_flux<64, !MIXED>(m, cC, (long long)p * 8 + ic0, fs, dtdx, gp, s_f, kin, t, upre, un);
+        cand = courant_cand_u(m, un, cC, dx, gp.gamma, courant_factor);
+    }
+    // One wave holds the workgroup's cells: its minimum is the block's.
+    if (cpart) {
+        cand = wave_min(cand);
+        if (t == 0) cpart[blockIdx.x] = cand;
+        return;
+    }
+    const int pfirst = praw - (t >> 3);
+    const int b0 = lv.obox[pfirst], b1 = lv.obox[min(pfirst + 8, noct) - 1];
+    if (b0 == b1) {
+        cand = wave_min(cand);
+        if (t == 0) atomicMin(&ckeys[b0], dt_key(cand));
+    } else if (cand != __builtin_inf()) {
+        atomicMin(&ckeys[lv.obox[p]], dt_key(cand));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Brick variant: OCTS consecutive octs of the list per workgroup, the cells
 // of an oct in eight adjacent lanes as above.  On a uniform level, whose
 // list is the Morton order of the octs, 64 / 32 / 16 consecutive octs are a
@@ -2129,8 +2385,13 @@ hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, dou
 }
 
 int godunov_blocks(const LevelDev& lv, int variant) {
-    const int octs = variant == 1 ? 16 : variant == 2 ? 8 : variant == 3 ? 16 : variant == 4 ? 32 : variant == 5 ? 64
-                                                                                                           : G_OCTS;
+    const int octs = variant == 1   ? 16
+                     : variant == 2 ? 8
+                     : variant == 3 ? 16
+                     : variant == 4 ? 32
+                     : variant == 5 ? 64
+                     : variant == 6 ? 8
+                                    : G_OCTS;
     return (lv.noct + octs - 1) / octs;
 }
 
@@ -2188,7 +2449,10 @@ template <int R, bool MIXED, bool EXACT>
 void launch_form(int variant, const MeshDev& m, const LevelDev& lv, const GasParams& gp, const double* d_dt,
                  double dx, double inv_dx, const FusedCourant& fc, hipStream_t stream) {
     const double* pre = fc.pre_from_uold ? m.uold : m.unew;
-    if (variant >= 3) {
+    if (variant == 6) {
+        hipLaunchKernelGGL((k_godunov_dims<R, MIXED, EXACT>), dim3((lv.noct + 7) / 8), dim3(192), 0, stream, m, lv,
+                           gp, d_dt, dx, inv_dx, pre, fc.part, fc.keys, fc.courant_factor);
+    } else if (variant >= 3) {
         const int octs = variant == 3 ? 16 : variant == 4 ? 32 : 64;
         const dim3 grid((lv.noct + octs - 1) / octs);
         if (variant == 3)

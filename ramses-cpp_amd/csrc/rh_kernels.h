@@ -95,20 +95,22 @@ hipError_t launch_step_join(const MeshDev& m, const LevelDev& lv, double dx, dou
 // 2: the eight cells of an oct in adjacent lanes, sharing by cross-lane
 // shuffles, 16 (variant 1) or 8 (variant 2) octs per workgroup; variants 3,
 // 4, 5: bricks of 16, 32, 64 consecutive octs per workgroup that share face
-// states and fluxes between octs through LDS as well.
+// states and fluxes between octs through LDS as well; variant 6: 8 octs per
+// workgroup of three waves, one per dimension (for levels too small to fill
+// the GPU).
 // dt is d_dt[0] for a single mesh and d_dt[box] for an ensemble.  inv_dx is
 // 1/dx when dx is a power of two (the FP64 form then multiplies by it, with
 // identical results), else 0; an ensemble reads dx and 1/dx per box and
 // passes inv_dx != 0 when every box has an exact reciprocal.
-constexpr int KERNEL_VARIANTS = 6;
+constexpr int KERNEL_VARIANTS = 7;
 // pre_from_uold: the launch reads the cells' pre-update unew from uold, for
 // a caller that knows unew equals uold on the level's leaf cells (what
-// set_unew copies; variants 1 to 5).  Fused step of a single-level run or an
+// set_unew copies; variants 1 to 6).  Fused step of a single-level run or an
 // ensemble: each leaf cell also takes the Courant candidate of its new state
 // (what set_uold copies into uold) into a block minimum in
 // part[godunov_blocks(lv, variant)] (single mesh, part non-null) or into
 // its box's key in keys (ensemble, keys non-null; as launch_courant_boxes),
-// only for variants 1 to 5 on levels with only leaf cells and same-level
+// only for variants 1 to 6 on levels with only leaf cells and same-level
 // neighbours (no records, no reflux targets).
 struct FusedCourant {
     bool pre_from_uold = false;

@@ -22,7 +22,7 @@ RH_HOST_COHERENT, RH_DEVICE_RESIDENT = 0, 1
 RH_SYNC_TO_HOST, RH_SYNC_TO_DEVICE = 1, 2
 RH_UOLD, RH_UNEW = 1, 2
 TK_GODUNOV, TK_COURANT, TK_SET_UNEW, TK_SET_UOLD, TK_STEP_JOIN = range(5)
-RH_KERNEL_VARIANTS = 6  # 0 per-cell; 1, 2 oct-in-lanes with 16 / 8 octs per workgroup; 3, 4, 5 bricks of 16 / 32 / 64 octs
+RH_KERNEL_VARIANTS = 7  # 0 per-cell; 1, 2 oct-in-lanes with 16 / 8 octs per workgroup; 3, 4, 5 bricks of 16 / 32 / 64 octs; 6 one wave per dimension
 
 _IP = C.POINTER(C.c_int)
 _DP = C.POINTER(C.c_double)
