@@ -30,25 +30,32 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 80  # read uold (5 x FP64) + write unew (5 x FP64)
 
 # Counter figures of the Godunov kernel from rocprofv3 runs of this script
-# (profiles/), keyed by (level, riemann, slope_type, kernel variant, form):
-#   traffic  HBM bytes per launch, from --pmc FETCH_SIZE and --pmc WRITE_SIZE
-#            runs, each counter scaled by the ratio it reads in the same run
-#            for k_set_unew_oct_lanes, a copy of known size with the Godunov
-#            kernel's lane mapping (x1.9987 for reads, x0.9698 for writes);
+# with --variant (profiles/), keyed by (level, riemann, slope_type, kernel
+# variant, form), for a launch with the next step's Courant candidates:
+#   traffic  HBM bytes per launch: FETCH_SIZE (one --pmc run) scaled by the
+#            ratio it reads in the same run for k_set_unew_oct_lanes, a copy
+#            of known size with the Godunov kernel's lane mapping (x1.998),
+#            plus WRITE_SIZE (another run) as read: on a call's last step,
+#            whose only writes are the 671,088,640 B of unew, it reads those
+#            bytes to within 0.03 % (the copy's own write count runs 2-5 %
+#            high and varies between runs);
 #   busy     VALU utilisation (SQ_ACTIVE_INST_VALU x 4 / SIMDs / busy cycles);
-#   insts_per_cell  VALU instructions per wave = per cell (one cell per lane),
-#            SQ_INSTS_VALU / SQ_WAVES, of a launch with or without the next
-#            step's Courant candidates (courant_epilogue).
+#   insts_per_cell  VALU instructions per cell, SQ_INSTS_VALU / SQ_WAVES x
+#            the waves per 64 cells (1, or 3 for variant 6).
 # Counters cannot be read while timing.  Each entry records the hash of the
 # kernel sources it was measured on (KERNEL_SOURCES); a build whose sources
 # hash differently reports null instead of a stale figure.
 KERNEL_SOURCES = ["ramses-cpp_amd/csrc/rh_hydro.hip", "ramses-cpp_amd/csrc/rh_physics.h",
                   "ramses-cpp_amd/csrc/rh_kernels.h", "ramses-cpp_amd/Makefile"]
 PROFILED = {
-    (9, "hllc", 1, 2, "fp64"): {"sources": "938a646e7aaf49b3", "traffic": 2153530546,
-                                "valu": {"busy": 0.766, "insts_per_cell": 5985.4, "courant_epilogue": True}},
-    (9, "hllc", 1, 2, "mixed"): {"sources": "938a646e7aaf49b3", "traffic": 2172086077,
-                                 "valu": {"busy": 0.866, "insts_per_cell": 4001.6, "courant_epilogue": True}},
+    (9, "hllc", 1, 2, "fp64"): {"sources": "6136272437820073", "traffic": 2172896217,
+                                "valu": {"busy": 0.766, "insts_per_cell": 5941.0, "courant_epilogue": True}},
+    (9, "hllc", 1, 6, "fp64"): {"sources": "6136272437820073", "traffic": 2175720772,
+                                "valu": {"busy": 0.821, "insts_per_cell": 6237.0, "courant_epilogue": True}},
+    (9, "hllc", 1, 2, "mixed"): {"sources": "6136272437820073", "traffic": 2195972482,
+                                 "valu": {"busy": 0.860, "insts_per_cell": 3958.2, "courant_epilogue": True}},
+    (9, "hllc", 1, 6, "mixed"): {"sources": "6136272437820073", "traffic": 2177957496,
+                                 "valu": {"busy": 0.690, "insts_per_cell": 4291.2, "courant_epilogue": True}},
 }
 
 
@@ -522,6 +529,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="Godunov kernel variant of the main workload (default: autotuned on the level)")
     ap.add_argument("--fp64-only", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--boxes", type=int, default=96, help="boxes of the shock-tube and Sedov ensemble (0: skip it)")
     ap.add_argument("--no-variants", action="store_true", help="skip the HLL / slope-2 variants of the workload")
@@ -569,14 +578,18 @@ def main() -> int:
         """Warmup, then exactly `steps` timed steps of one precision form."""
         mesh.uold[:] = u0
         mesh.unew[:] = u0
-        s = rhydro.Solver(gamma=gamma, slope_type=slope, riemann=riemann, precision=precision)
+        s = rhydro.Solver(gamma=gamma, slope_type=slope, riemann=riemann, precision=precision,
+                          variant=args.variant)
         s.bind(mesh, rhydro.RH_DEVICE_RESIDENT)
         nleaf = s.leaf_cells(level)
         # Pick the faster Godunov kernel variant on this level (results are
-        # identical for every variant; unew is restored by the tuner), then
-        # the warmup steps right before the timed ones, so that no host work
-        # leaves the GPU idle in between.
-        best, tune_ms = s.autotune(level, s.courant(level, dx, gamma, cf), dx, reps=2)
+        # identical for every variant; unew is restored by the tuner) unless
+        # one is given, then the warmup steps right before the timed ones, so
+        # that no host work leaves the GPU idle in between.
+        if args.variant is None:
+            best, tune_ms = s.autotune(level, s.courant(level, dx, gamma, cf), dx, reps=2)
+        else:
+            best, tune_ms = args.variant, np.full(rhydro.RH_KERNEL_VARIANTS, np.nan)
         s.run_uniform(level, dx, cf, max(args.warmup, 1))  # ends with a stream synchronisation
         s.timers(True)
         s.timers_reset()
@@ -612,7 +625,8 @@ def main() -> int:
               f"eager {untimed['eager']:.3f} / graph {untimed['graph']:.3f} ms per step", file=sys.stderr, flush=True)
         return dict(nleaf=nleaf, wall=wall, dts=dts, achieved=achieved, final=final,
                     kernel_ms={n: v[0] / max(v[1], 1) for n, v in k.items()},
-                    autotune={"chosen_variant": best, "godunov_ms_by_variant": tune_ms.tolist()},
+                    autotune={"chosen_variant": best, "given": args.variant is not None,
+                              "godunov_ms_by_variant": tune_ms.tolist()},
                     untimed_ms_per_step=untimed)
 
     fp64 = measure(rhydro.RH_PREC_FP64)

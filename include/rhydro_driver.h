@@ -92,8 +92,11 @@ int rh_set_kernel_variant(rh_ctx* ctx, int variant);
 int rh_get_kernel_variant(rh_ctx* ctx, int* variant);
 
 /* Times each Godunov kernel variant on level `ilevel` of the bound mesh
- * (reps timed launches after one untimed launch, HIP events on the
- * context's stream), keeps the fastest for the context and reports it in
+ * (eight warm-up launches, then per variant four untimed launches and `reps`
+ * timed ones back to back, HIP events on the context's stream around each;
+ * the launches read the pre-update state from uold and, on a uniform
+ * periodic level, end with the next step's Courant candidates, as the
+ * multi-step calls do), keeps the fastest for the context and reports it in
  * *best_variant and the average ms per launch in
  * ms_per_variant[RH_KERNEL_VARIANTS] (either pointer may be NULL).  unew is
  * restored after every trial. */

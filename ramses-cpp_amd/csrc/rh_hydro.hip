@@ -186,17 +186,25 @@ RH_DEV double courant_cand_u(const MeshDev& m, const double u[5], int id, double
         double v = u[i] / d;
         v2 = v2 + v * v;
     }
-    double sum_gamma_p = 0.0;
-    double e_nonthermal = 0.0;
-    double pr = smax((u[4] - 0.5 * d * v2 - e_nonthermal) * (gamma - 1.0), d * 1e-10);
+    // The reference subtracts a non-thermal energy of 0 and adds a
+    // non-thermal sound speed of 0/d: both leave every value unchanged (c2
+    // is at least 1e-20 or NaN, and d is NaN only where the velocities and
+    // so the result are), so they are left out.
+    double pr = smax((u[4] - 0.5 * d * v2) * (gamma - 1.0), d * 1e-10);
     pr = smin(pr, d * 1e2);
     double c2 = smax(gamma * pr / smax(d, 1e-10), 1e-20);
-    c2 = c2 + sum_gamma_p / d;
     double cs = ssqrt(smax(c2, 1e-10));
     double u_wave = 3 * cs;
     for (int i = 1; i <= 3; ++i) u_wave = u_wave + sabs(u[i] / d);
+    if (!m.f) {
+        // No gravity: a_grav = 0, so alpha = max(0 * dx / u_wave^2, 1e-4) =
+        // 1e-4 wherever u_wave is a number (it is at least 3e-5); where it
+        // is NaN the result is NaN either way.
+        const double s1 = ssqrt(1.0 + 2.0 * courant_factor * 1e-4) - 1.0;
+        return (dx / u_wave) * s1 / 1e-4;
+    }
     double a_grav = 0.0;
-    for (int i = 1; i <= 3; ++i) a_grav = a_grav + sabs(m.f ? m.f[(size_t)(i - 1) * m.ncell + id - 1] : 0.0);
+    for (int i = 1; i <= 3; ++i) a_grav = a_grav + sabs(m.f[(size_t)(i - 1) * m.ncell + id - 1]);
     double alpha = a_grav * dx / (u_wave * u_wave);
     alpha = smax(alpha, 0.0001);
     return (dx / u_wave) * (ssqrt(1.0 + 2.0 * courant_factor * alpha) - 1.0) / alpha;
