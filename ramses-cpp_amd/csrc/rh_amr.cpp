@@ -942,7 +942,8 @@ int amr_step(rh_amr* a, int il, int icount) {
     const double dx = a->boxlen / (double)(a->nx * (1 << il));
     // dtold/dtnew/t live in d_times: the Courant minimum, the subcycling
     // limit and the time update are stream-ordered on the device.
-    const hipStream_t stream = a->ctx->stream;
+    const int nsub = il < (int)a->nsubcycle.size() ? a->nsubcycle[il] : 1;
+    const bool has_finer = il < a->nlevelmax && count(a, il + 1) > 0;
     {
         Phase ph{a, PH_HYDRO};
         // The Courant pass also does the level's set_unew (it copies the
@@ -953,18 +954,18 @@ int amr_step(rh_amr* a, int il, int icount) {
         at.il = il;
         at.lmin = a->levelmin;
         at.nsub_prev = (double)a->nsubcycle[il - 1];
+        // A level without finer octs takes no sub-steps: the bookkeeping that
+        // ends its step (op 1, or op 2 on the finest level) follows op 0 in
+        // the same launch.
+        if (!has_finer) {
+            at.then_op = il < a->nlevelmax ? 1 : 2;
+            at.then_nsub = il < a->nlevelmax ? (double)nsub : 0.0;
+        }
         RH_ACHK(courant_dev(a->ctx, il, dx, a->gamma, a->ap.courant_factor, a->d_times + rh::AMR_T_DTC, true, at));
     }
-    const int nsub = il < (int)a->nsubcycle.size() ? a->nsubcycle[il] : 1;
-    if (il < a->nlevelmax) {
-        if (count(a, il + 1) > 0) {
-            for (int i = 1; i <= nsub; ++i) RH_ACHK(amr_step(a, il + 1, i));
-        } else {
-            RH_AHIP(a, rh::launch_amr_times(a->d_times, 1, il, a->levelmin, 0.0, (double)nsub, stream));
-            a->nstep++;
-        }
+    if (has_finer) {
+        for (int i = 1; i <= nsub; ++i) RH_ACHK(amr_step(a, il + 1, i));
     } else {
-        RH_AHIP(a, rh::launch_amr_times(a->d_times, 2, il, a->levelmin, 0.0, 0.0, stream));
         a->nstep++;
     }
     {

@@ -407,7 +407,8 @@ __global__ __launch_bounds__(256) void k_keys_to_dt(const unsigned long long* ke
 RH_DEV void amr_times(double* T, int op, int il, int lmin, double nsub_prev, double nsub);
 
 // at.T non-null: after storing the minimum, the adaptive run's time-step
-// bookkeeping of the level step (op 0 of k_amr_times) in the same thread.
+// bookkeeping of the level step (op 0 of k_amr_times, then at.then_op when
+// set) in the same thread.
 __global__ __launch_bounds__(1024) void k_courant_final(const double* part, int npart, double dt_init, double* dt,
                                                         AmrTimesArgs at) {
     __shared__ double s_min[16];
@@ -429,7 +430,10 @@ __global__ __launch_bounds__(1024) void k_courant_final(const double* part, int 
         double r = dt_init;
         for (int w = 0; w < 16; ++w) r = min_skip_nan(r, s_min[w]);
         *dt = r;
-        if (at.T) amr_times(at.T, 0, at.il, at.lmin, at.nsub_prev, 0.0);
+        if (at.T) {
+            amr_times(at.T, 0, at.il, at.lmin, at.nsub_prev, 0.0);
+            if (at.then_op) amr_times(at.T, at.then_op, at.il, at.lmin, 0.0, at.then_nsub);
+        }
     }
 }
 
@@ -2412,7 +2416,8 @@ hipError_t launch_courant_final(const double* d_part, int nparts, double dt_init
 hipError_t launch_courant(const MeshDev& m, const LevelDev& lv, double dx, double gamma, double courant_factor,
                           double dt_init, double* d_part, int npart, double* d_dt, hipStream_t stream,
                           bool set_unew, const AmrTimesArgs& at) {
-    if (at.T && (at.il < 1 || at.il + 1 >= AMR_T_LEVELS)) return hipErrorInvalidValue;
+    if (at.T && (at.il < 1 || at.il + 1 >= AMR_T_LEVELS || at.then_op < 0 || at.then_op > 2))
+        return hipErrorInvalidValue;
     int nb = stream_blocks(lv.noct);
     if (nb > npart) return hipErrorInvalidValue;
     if (nb > 0 && set_unew)

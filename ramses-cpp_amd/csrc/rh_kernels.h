@@ -67,6 +67,11 @@ struct AmrTimesArgs {
     double* T = nullptr;
     int il = 0, lmin = 0;
     double nsub_prev = 1.0;
+    // then_op 1 or 2: the level has no finer octs, so the bookkeeping's op
+    // that ends its step (1 below the finest level, with then_nsub
+    // sub-steps; 2 on it) runs right after op 0, in the same thread.
+    int then_op = 0;
+    double then_nsub = 0.0;
 };
 // Courant minimum into *d_dt (device).  d_part has >= courant_blocks(lv) doubles.
 // set_unew: the same pass also does set_unew (uold -> unew on every cell of
