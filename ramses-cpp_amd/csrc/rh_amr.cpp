@@ -948,7 +948,7 @@ int amr_step(rh_amr* a, int il, int icount) {
         Phase ph{a, PH_HYDRO};
         // The Courant pass also does the level's set_unew (it copies the
         // uold values the Courant step reads), and its reduction the time
-        // step bookkeeping (op 0 of k_amr_times).
+        // step bookkeeping (op 0 of amr_times).
         rh::AmrTimesArgs at;
         at.T = a->d_times;
         at.il = il;

@@ -17,9 +17,9 @@
 //                             cross-lane shuffle, ordered accumulation in LDS)
 //   k_coarse_reflux(_wave)    ordered reflux into coarser leaf cells
 //   k_interpol / k_new_octs   prolongation of coarse cells into new octs
-//   k_restrict, k_set_uold_restrict, k_amr_times
-//                             adaptive-run restriction and subcycling
-//                             time-step bookkeeping
+//   k_restrict, k_set_uold_restrict
+//                             adaptive-run restriction (the subcycling
+//                             time-step bookkeeping runs in k_courant_final)
 //   k_smooth_bits, k_grad_bits
 //                             flag_fine's smoothing passes and gradient
 //                             criteria over a level list
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void k_keys_to_dt(const unsigned long long* ke
 RH_DEV void amr_times(double* T, int op, int il, int lmin, double nsub_prev, double nsub);
 
 // at.T non-null: after storing the minimum, the adaptive run's time-step
-// bookkeeping of the level step (op 0 of k_amr_times, then at.then_op when
+// bookkeeping of the level step (op 0 of amr_times, then at.then_op when
 // set) in the same thread.
 __global__ __launch_bounds__(1024) void k_courant_final(const double* part, int npart, double dt_init, double* dt,
                                                         AmrTimesArgs at) {
@@ -2103,7 +2103,8 @@ __global__ __launch_bounds__(64) void k_new_octs(MeshDev m, int n, const int* __
 
 // Time-step bookkeeping of the adaptive run's subcycling (Simulation.cpp:
 // 522-635) on the device, so that no fine step waits on the host for its
-// Courant step.  One thread; T as laid out by AMR_T_* in rh_kernels.h.
+// Courant step; run by k_courant_final's first thread.  T as laid out by
+// AMR_T_* in rh_kernels.h.
 // op 0: after the level's Courant step (its minimum in T[AMR_T_DTC]):
 //   dtold[il] = dtnew[il]; dtnew[il] = min(dtc, dtnew[il-1]/nsub_prev) for
 //   il > lmin, else dtc (std::min's operand order); used[il] = dtnew[il].
@@ -2130,12 +2131,6 @@ RH_DEV void amr_times(double* T, int op, int il, int lmin, double nsub_prev, dou
         }
         T[AMR_T_TIME] = T[AMR_T_TIME] + used[il];
     }
-}
-
-__global__ __launch_bounds__(64) void k_amr_times(double* T, int op, int il, int lmin, double nsub_prev,
-                                                  double nsub) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    amr_times(T, op, il, lmin, nsub_prev, nsub);
 }
 
 // restrict_fine (TreeUpdater.cpp:195-215): the father cell of every oct of
@@ -2539,13 +2534,6 @@ hipError_t launch_new_octs(const MeshDev& m, int n, const int* d_cells, const Ga
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_new_octs, dim3((n + 63) / 64), dim3(64), 0, stream, m, n, d_cells, gp, interpol_var,
                        d_father, d_nbor);
-    return hipGetLastError();
-}
-
-hipError_t launch_amr_times(double* d_T, int op, int il, int lmin, double nsub_prev, double nsub,
-                            hipStream_t stream) {
-    if (il < 1 || il + 1 >= AMR_T_LEVELS) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_amr_times, dim3(1), dim3(64), 0, stream, d_T, op, il, lmin, nsub_prev, nsub);
     return hipGetLastError();
 }
 

@@ -61,8 +61,8 @@ hipError_t launch_set_unew_oct_lanes(const MeshDev& m, const LevelDev& lv, hipSt
 hipError_t launch_gravity_source(const MeshDev& m, const LevelDev& lv, double dt, hipStream_t stream);
 hipError_t launch_synchro(const MeshDev& m, const LevelDev& lv, double dt, double gamma, hipStream_t stream);
 // An adaptive run's level step whose time-step bookkeeping (op 0 of
-// launch_amr_times on table T) the Courant reduction does once its minimum
-// is stored; T null: none.
+// amr_times in rh_hydro.hip, on table T) the Courant reduction does once its
+// minimum is stored; T null: none.
 struct AmrTimesArgs {
     double* T = nullptr;
     int il = 0, lmin = 0;
@@ -159,10 +159,6 @@ constexpr int AMR_T_USED = 2 * AMR_T_LEVELS;
 constexpr int AMR_T_TIME = 3 * AMR_T_LEVELS;
 constexpr int AMR_T_DTC = 3 * AMR_T_LEVELS + 1;
 constexpr int AMR_T_SIZE = 3 * AMR_T_LEVELS + 2;
-// One bookkeeping step (op 0: after a Courant step; 1: a level without finer
-// octs; 2: the finest level), as k_amr_times in rh_hydro.hip describes.
-hipError_t launch_amr_times(double* d_T, int op, int il, int lmin, double nsub_prev, double nsub,
-                            hipStream_t stream);
 
 // flag_fine's passes over a level list on the device (flags: one byte per
 // oct in list order, bit ic-1 for cell ic): one smooth_fine pass, the
