@@ -48,14 +48,14 @@ BYTES_PER_CELL_UPDATE = 80  # read uold (5 x FP64) + write unew (5 x FP64)
 KERNEL_SOURCES = ["ramses-cpp_amd/csrc/rh_hydro.hip", "ramses-cpp_amd/csrc/rh_physics.h",
                   "ramses-cpp_amd/csrc/rh_kernels.h", "ramses-cpp_amd/Makefile"]
 PROFILED = {
-    (9, "hllc", 1, 2, "fp64"): {"sources": "e9d4d951f66ec39f", "traffic": 2171156030,
-                                "valu": {"busy": 0.764, "insts_per_cell": 5941.0, "courant_epilogue": True}},
-    (9, "hllc", 1, 6, "fp64"): {"sources": "e9d4d951f66ec39f", "traffic": 2169198676,
-                                "valu": {"busy": 0.818, "insts_per_cell": 6237.0, "courant_epilogue": True}},
-    (9, "hllc", 1, 2, "mixed"): {"sources": "e9d4d951f66ec39f", "traffic": 2191356375,
-                                 "valu": {"busy": 0.860, "insts_per_cell": 3958.2, "courant_epilogue": True}},
-    (9, "hllc", 1, 6, "mixed"): {"sources": "e9d4d951f66ec39f", "traffic": 2172146726,
-                                 "valu": {"busy": 0.688, "insts_per_cell": 4291.2, "courant_epilogue": True}},
+    (9, "hllc", 1, 2, "fp64"): {"sources": "aaea3b054087f57a", "traffic": 2172901455,
+                                "valu": {"busy": 0.761, "insts_per_cell": 5941.0, "courant_epilogue": True}},
+    (9, "hllc", 1, 6, "fp64"): {"sources": "aaea3b054087f57a", "traffic": 2175126472,
+                                "valu": {"busy": 0.812, "insts_per_cell": 6237.0, "courant_epilogue": True}},
+    (9, "hllc", 1, 2, "mixed"): {"sources": "aaea3b054087f57a", "traffic": 2197427939,
+                                 "valu": {"busy": 0.849, "insts_per_cell": 3958.2, "courant_epilogue": True}},
+    (9, "hllc", 1, 6, "mixed"): {"sources": "aaea3b054087f57a", "traffic": 2178278029,
+                                 "valu": {"busy": 0.685, "insts_per_cell": 4291.2, "courant_epilogue": True}},
 }
 
 
