@@ -145,3 +145,30 @@ def test_checkerboard_refinement_many_reflux_targets(variant):
     assert_same(mg.unew, mo.unew, "unew")
     assert_same(mg.uold, mo.uold, "uold")
     s.close()
+
+
+def test_autotune_on_refined_levels_leaves_state_and_results():
+    """rh_autotune on levels with coarser neighbours and refined cells (its
+    trials without the Courant epilogue, pre-update state from uold) leaves
+    uold and unew as they were, and the tuned context still equals the
+    oracle value for value."""
+    m0 = refined_mesh(4, SPHERES, riemann=rhydro.resolve_riemann("hllc"))
+    levels = [6, 5, 4]
+    dts = [2e-3, 4e-3, 8e-3]
+    mg, mo = clone(m0), clone(m0)
+    s = rhydro.Solver(gamma=1.4, slope_type=1, riemann="hllc")
+    s.bind(mg, rhydro.RH_DEVICE_RESIDENT)
+    for L in (6, 5):
+        print(f"case autotune amr level {L} octs={m0.numbl.tolist()}", flush=True)
+        best, ms = s.autotune(L, 2e-3, 1.0 / 2 ** L, reps=1)
+        assert best in range(rhydro.RH_KERNEL_VARIANTS) and s.variant() == best
+        assert (ms > 0).all()
+        s.sync_to_host()
+        assert_same(mg.uold, m0.uold, "uold after autotune")
+        assert_same(mg.unew, m0.unew, "unew after autotune")
+    _sequence(s, levels, dts)
+    s.sync_to_host()
+    _sequence(_OracleApi(mo, oracle.params(1.4, 1, rhydro.resolve_riemann("hllc"))), levels, dts)
+    assert_same(mg.unew, mo.unew, "unew")
+    assert_same(mg.uold, mo.uold, "uold")
+    s.close()
