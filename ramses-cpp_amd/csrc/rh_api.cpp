@@ -835,7 +835,11 @@ int rh_autotune(rh_ctx* c, int ilevel, double dt, double dx, int reps, int* best
         const size_t bytes = state_bytes(c);
         double* saved = nullptr;
         RH_HIP(c, hipMalloc(&saved, bytes));
-        RH_HIP(c, hipMemcpyAsync(saved, c->d_unew, bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (const hipError_t e = hipMemcpyAsync(saved, c->d_unew, bytes, hipMemcpyDeviceToDevice, c->stream);
+            e != hipSuccess) {
+            (void)hipFree(saved);
+            return hip_fail(c, e, "save unew");
+        }
         constexpr int NV = RH_KERNEL_VARIANTS;
         static_assert(NV == rh::KERNEL_VARIANTS, "variant count of the C-ABI and the kernels");
         // Each trial launches as the multi-step runs do: the pre-update state
