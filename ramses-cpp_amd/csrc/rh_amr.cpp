@@ -546,41 +546,66 @@ void ensure_ref_rules(rh_amr* a, const std::vector<int>& grids) {
         const int fc = a->father[ig - 1];
         return fc > a->ncoarse ? ((fc - a->ncoarse - 1) % a->ngridmax) + 1 : 0;
     };
-    std::vector<int> start;
-    start.reserve(grids.size() / 4 + 2);
-    for (size_t k = 0; k < grids.size(); ++k)
-        if (k == 0 || father_oct(grids[k]) != father_oct(grids[k - 1]) || father_oct(grids[k]) == 0)
-            start.push_back((int)k);
-    start.push_back((int)grids.size());
-    // A run costs 64 neighbour walks and, next to a coarser region, up to
-    // eight coordinate searches (microseconds), so the worker threads take
-    // lists of a few dozen runs already.
-    parallel_for(
-        (int)start.size() - 1,
-        [&](int r) {
-            int block[64];
-            bool have = false;
-            for (int k = start[r]; k < start[r + 1]; ++k) {
-                const int ig = grids[k];
-                const int fc = a->father[ig - 1];
-                bool ok;
-                if (fc <= 0) {
-                    ok = false;
-                } else if (fc <= a->ncoarse || !cubic) {
-                    ok = neighbours_refined_by_search(a, fc);
-                } else {
-                    const int g = ((fc - a->ncoarse - 1) % a->ngridmax) + 1;
-                    if (!have) {
-                        cells_around(a, g, block);
-                        have = true;
-                    }
-                    ok = neighbours_refined_in_block(a, g, (fc - a->ncoarse - 1) / a->ngridmax, block);
+    auto run_start = [&](size_t k) {
+        return k == 0 || father_oct(grids[k]) != father_oct(grids[k - 1]) || father_oct(grids[k]) == 0;
+    };
+    // The octs [k, end) of one run.
+    auto do_run = [&](size_t k, size_t end) {
+        int block[64];
+        bool have = false;
+        for (size_t j = k; j < end; ++j) {
+            const int ig = grids[j];
+            const int fc = a->father[ig - 1];
+            bool ok;
+            if (fc <= 0) {
+                ok = false;
+            } else if (fc <= a->ncoarse || !cubic) {
+                ok = neighbours_refined_by_search(a, fc);
+            } else {
+                const int g = ((fc - a->ncoarse - 1) % a->ngridmax) + 1;
+                if (!have) {
+                    cells_around(a, g, block);
+                    have = true;
                 }
-                if (!ok)
-                    for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
+                ok = neighbours_refined_in_block(a, g, (fc - a->ncoarse - 1) / a->ngridmax, block);
+            }
+            if (!ok)
+                for (int ic = 1; ic <= 8; ++ic) a->flag1[cell(a, ig, ic) - 1] = 0;
+        }
+    };
+    // Each oct's flags depend on the tree only, so the runs go to the worker
+    // threads in any grouping.
+    const size_t n = grids.size();
+    if (n < (1u << 16)) {
+        // A run costs 64 neighbour walks and, next to a coarser region, up
+        // to eight coordinate searches (microseconds), so the worker threads
+        // take lists of a few dozen runs already.
+        std::vector<int> start;
+        start.reserve(n / 4 + 2);
+        for (size_t k = 0; k < n; ++k)
+            if (run_start(k)) start.push_back((int)k);
+        start.push_back((int)n);
+        parallel_for((int)start.size() - 1, [&](int r) { do_run(start[r], start[r + 1]); }, 32);
+        return;
+    }
+    // Long lists: finding the runs is itself split over the threads.  The
+    // list is cut into chunks; a chunk takes the runs that start in it, to
+    // their ends (which may lie in the next chunk).
+    const int nchunks = (int)std::min<size_t>(n / 128, 8192);
+    parallel_for(
+        nchunks,
+        [&](int c) {
+            const size_t lo = n * (size_t)c / (size_t)nchunks, hi = n * (size_t)(c + 1) / (size_t)nchunks;
+            size_t k = lo;
+            while (k < hi && !run_start(k)) ++k;
+            while (k < hi) {
+                size_t end = k + 1;
+                while (end < n && !run_start(end)) ++end;
+                do_run(k, end);
+                k = end;
             }
         },
-        32);
+        2);
 }
 
 // Gradient criteria of the coarse cells (TreeUpdater.cpp:459-480), on the
